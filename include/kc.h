@@ -1,0 +1,206 @@
+/*
+ * kc.h -- C ABI of the MI355X (gfx950) exact k-clique counter.
+ *
+ * The reference (KCGPU) has no library interface: its path is reached through
+ * the CLI in src/main.cu and the C++ templates of graph::SingleGPU_Kclique
+ * (kclique/kclique.cuh:1302-2643).  Each entry point below names the
+ * reference code it replaces.  Plain C types only; every call returns an int
+ * status (KC_OK = 0) and never exits the process; kc_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Handles are opaque.  A handle is not thread-safe; distinct handles are.
+ * All calls are synchronous: when a call returns, its device work is done.
+ */
+#ifndef KC_H
+#define KC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define KC_OK            0
+#define KC_ERR_ARG       1  /* invalid argument (bad k, bad -q spec, null) */
+#define KC_ERR_IO        2  /* file missing or unreadable */
+#define KC_ERR_FORMAT    3  /* malformed file, unknown extension */
+#define KC_ERR_DEVICE    4  /* HIP runtime error, no usable gfx950 device */
+#define KC_ERR_NOMEM     5  /* host or device allocation failed */
+#define KC_ERR_OVERFLOW  6  /* the count is >= 2^64; result.overflow is set */
+#define KC_ERR_BUDGET    7  /* admission refused: footprint above budget */
+#define KC_ERR_NOTFOUND  8  /* unknown graph id / ticket */
+#define KC_ERR_LIMIT     9  /* a fixed limit of this build was exceeded */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *kc_last_error(void);
+
+/* ---- graphs ------------------------------------------------------------ */
+typedef struct kc_graph kc_graph;        /* simple undirected graph (host CSR) */
+typedef struct kc_oriented kc_oriented;  /* oriented CSR resident in HBM */
+
+typedef struct {
+    uint32_t n;           /* vertices (max id + 1) */
+    uint64_t nnz;         /* CSR entries = 2 * undirected edges */
+    uint32_t max_degree;
+    uint64_t checksum;    /* FNV-1a 64 over rowptr[0..n] then col[0..nnz) as LE u32 */
+} kc_graph_info;
+
+/* kc_load flags */
+#define KC_LOAD_SORT 1u   /* -s: accepted for drop-in; the builder always sorts */
+
+/*
+ * Load an edge list file and build the simple undirected CSR.
+ * Replaces graph::EdgeListFile (include/FIleReader.cuh:33-476) plus
+ * CSRCOO::from_edgelist (include/CSRCOO.cuh:37-80) and src/main.cu:73-131.
+ * The extension selects the format as in the reference (.mtx, .tsv/.txt,
+ * .bel).  Self-loops are dropped, every edge is stored in both directions and
+ * duplicates are removed (the reference keeps duplicates and, for symmetric
+ * MTX, only one direction of each edge).  Ids are kept as written (MTX stays
+ * 1-based, so vertex 0 is isolated), n = max id + 1.
+ */
+int kc_load(const char *path, unsigned flags, kc_graph **out);
+
+/*
+ * Raw edge list exactly as the reference's read loop produces it
+ * (src/main.cu:84-87 over EdgeListFile::get_edges with batches of 100),
+ * including its defects: symmetric MTX keeps (col,row) and leaves a stale
+ * slot per line, TSV emits both directions per line, BEL is not symmetrized.
+ * For documentation tests of the reference's behaviour.  On success *src and
+ * *dst are malloc'ed arrays of *count entries; release them with kc_free_buffer.
+ */
+int kc_read_edges_reference(const char *path, uint32_t **src, uint32_t **dst, size_t *count);
+void kc_free_buffer(void *p);
+
+/* Build from an edge list (both arrays of length nedges; n = 0: max id + 1). */
+int kc_graph_from_edges(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst,
+                        kc_graph **out);
+
+/*
+ * Build from a CSR (rowptr[n+1], col[rowptr[n]]).  The library copies; the
+ * caller keeps its arrays.  The CSR is cleaned like an edge list (symmetrized,
+ * deduplicated, self-loops dropped).  Replaces the host half of
+ * COOCSRGraph / to_csrcoo_device (include/GraphDataStructure.cuh:6-62).
+ */
+int kc_graph_from_csr(uint32_t n, const uint32_t *rowptr, const uint32_t *col, kc_graph **out);
+
+int kc_graph_get_info(const kc_graph *g, kc_graph_info *info);
+/* Copy the CSR out: rowptr needs n+1 entries, col needs nnz entries. */
+int kc_graph_get_csr(const kc_graph *g, uint32_t *rowptr, uint32_t *col);
+void kc_graph_free(kc_graph *g);
+
+/* ---- seeded synthetic graphs ------------------------------------------- */
+#define KC_GEN_ER        0  /* G(n, m): uniform endpoints, loops/duplicates rejected */
+#define KC_GEN_CHUNG_LU  1  /* power law: w_i ~ (i+i0)^(-1/(gamma-1)), max expected degree = max_degree */
+#define KC_GEN_RMAT      2  /* R-MAT scale/edge_factor/(a,b,c): 2^scale*edge_factor samples */
+#define KC_GEN_BA        3  /* Barabasi-Albert: n vertices, attach edges per new vertex */
+
+typedef struct {
+    int kind;
+    uint32_t n;            /* ER, CL, BA */
+    uint64_t m;            /* ER, CL: unique undirected edges */
+    double gamma;          /* CL degree exponent */
+    double max_degree;     /* CL max expected degree */
+    uint32_t scale;        /* RMAT */
+    uint32_t edge_factor;  /* RMAT */
+    double a, b, c;        /* RMAT quadrant probabilities (d = 1-a-b-c) */
+    uint32_t attach;       /* BA */
+    uint64_t seed;
+} kc_gen_params;
+
+/* Generators are deterministic in (params, seed) and independent of thread count. */
+int kc_generate(const kc_gen_params *p, kc_graph **out);
+
+/* ---- orientation ------------------------------------------------------- */
+#define KC_ORIENT_DEGREE 0  /* -o degree: (deg u, u) < (deg v, v) */
+#define KC_ORIENT_DEGEN  1  /* -o degen: (peel round u, u) < (peel round v, v) */
+
+typedef struct {
+    uint32_t n;
+    uint64_t m;               /* oriented edges */
+    uint32_t max_out_degree;
+    uint32_t max_core;        /* degen only: the reference's "Max Core" */
+    double preprocess_ms;     /* device time of upload-free orientation (HIP events) */
+} kc_oriented_info;
+
+/*
+ * Orient on the device: degree key or k-core peel rounds, keep flags, per-row
+ * kept counts, exclusive scan, stable per-row compaction.  Replaces
+ * src/main.cu:164-235 with SingleGPU_Kcore::findKcoreIncremental_async
+ * (kcore/kcore.cuh:442-584), init (include/main_support.cuh:54-128),
+ * CUBSelect / CUBScanExclusive (include/cub_wrappers.cuh:11-179) and
+ * warp_detect_deleted_edges (include/main_support.cuh:133-167).
+ * The oriented CSR stays in HBM inside the returned handle.
+ */
+int kc_orient(const kc_graph *g, int mode, kc_oriented **out);
+int kc_oriented_get_info(const kc_oriented *o, kc_oriented_info *info);
+/* Copy the oriented CSR (rowptr[n+1], col[m]) and the orientation key
+ * (degree or peel round, key[n]) to the host; any pointer may be NULL. */
+int kc_oriented_export(const kc_oriented *o, uint32_t *rowptr, uint32_t *col, uint32_t *key);
+void kc_oriented_free(kc_oriented *o);
+
+/* ---- counting ---------------------------------------------------------- */
+#define KC_ALGO_ORIENT 0  /* -q o... */
+#define KC_ALGO_PIVOT  1  /* -q p... */
+#define KC_GRAN_NODE   0  /* -p node */
+#define KC_GRAN_EDGE   1  /* -p edge */
+
+typedef struct {
+    uint32_t k;          /* clique size, 1..KC_MAX_K */
+    uint32_t algo;       /* KC_ALGO_* */
+    uint32_t gran;       /* KC_GRAN_* */
+    uint32_t part_size;  /* lanes per partition: 1,2,4,8,16,32,64 */
+    uint32_t binary;     /* 'b' = 1, 'n' = 0; both run the bitmap kernels */
+    uint32_t auto_tune;  /* 1: choose algo/gran/part_size from the graph */
+} kc_query;
+
+#define KC_MAX_K 32
+
+typedef struct {
+    uint64_t count;       /* exact k-clique count (valid when overflow == 0) */
+    uint32_t overflow;    /* 1 when the count does not fit in 64 bits */
+    uint32_t algo, gran, part_size;  /* what ran (after auto-tune) */
+    uint64_t tasks;       /* node or edge tasks that passed the filter */
+    uint64_t alg_bytes;   /* algorithmic HBM bytes of those tasks */
+    uint64_t tree_nodes;  /* DFS / SCT tree nodes visited */
+    double count_ms;      /* device time of the counting kernels (HIP events) */
+    double dominant_ms;   /* device time of the longest counting kernel */
+    uint64_t dominant_bytes; /* algorithmic bytes of that kernel's tasks */
+    uint32_t dominant_class; /* its size class (0: <=64, 1: <=128, 2: <=256, 3: <=512, 4: <=1024, 5: larger) */
+} kc_result;
+
+/*
+ * Count k-cliques of the oriented graph.  Replaces the eight
+ * SingleGPU_Kclique::findKclqueIncremental_{node,edge}_{binary_,pivot_,,nobin_pivot_}async<P>
+ * drivers (kclique/kclique.cuh:1448-2623) and the dispatch of
+ * src/main.cu:304-576.  Returns KC_ERR_OVERFLOW (with result->overflow = 1)
+ * when the count is 2^64 or more.
+ */
+int kc_count(const kc_oriented *o, const kc_query *q, kc_result *result);
+
+/*
+ * Parse the reference's -q grammar: [o|p]<P>[b|n|h] (include/Config.h:235-271).
+ * Unlike the reference, a spec of two characters ("p8") is parsed too (the
+ * reference silently ignores specs shorter than 3 characters) and an invalid
+ * spec is an error instead of a silent default.  k, gran and auto_tune are
+ * left untouched.
+ */
+int kc_parse_spec(const char *spec, kc_query *q);
+
+/* ---- device ------------------------------------------------------------ */
+typedef struct {
+    char name[64];
+    char arch[32];
+    int cu_count;
+    uint64_t hbm_bytes;
+    uint64_t lds_per_block;
+    int clock_khz;
+} kc_device_info;
+
+int kc_get_device_info(kc_device_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KC_H */

@@ -1,0 +1,267 @@
+"""kcgpu_amd -- Python host mirror of the kcgpu C ABI (include/kc.h).
+
+The compute path is the native library ``libkcgpu.so`` (HIP kernels for
+gfx950 behind a C ABI); this module only binds it with ctypes.  There is no
+Python or CPU fallback: importing fails loudly when the library is missing.
+
+The names follow the reference's drop-in surface:
+``Graph.load`` (EdgeListFile + CSRCOO), ``Graph.orient`` (-o degree|degen),
+``Oriented.count`` (-m kc with -k, -p node|edge, -q [o|p]<P>[b|n]).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkcgpu.so")
+
+KC_OK = 0
+KC_ERR_ARG = 1
+KC_ERR_IO = 2
+KC_ERR_FORMAT = 3
+KC_ERR_DEVICE = 4
+KC_ERR_NOMEM = 5
+KC_ERR_OVERFLOW = 6
+KC_ERR_BUDGET = 7
+KC_ERR_NOTFOUND = 8
+KC_ERR_LIMIT = 9
+
+GEN_ER, GEN_CHUNG_LU, GEN_RMAT, GEN_BA = 0, 1, 2, 3
+ORIENT = {"degree": 0, "degen": 1}
+ALGO = {"orient": 0, "pivot": 1}
+GRAN = {"node": 0, "edge": 1}
+
+
+class KCError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kcgpu error {code}: {msg}")
+        self.code = code
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("nnz", ctypes.c_uint64), ("max_degree", ctypes.c_uint32),
+                ("checksum", ctypes.c_uint64)]
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_uint32), ("m", ctypes.c_uint64),
+                ("gamma", ctypes.c_double), ("max_degree", ctypes.c_double), ("scale", ctypes.c_uint32),
+                ("edge_factor", ctypes.c_uint32), ("a", ctypes.c_double), ("b", ctypes.c_double),
+                ("c", ctypes.c_double), ("attach", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+class OrientedInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint64), ("max_out_degree", ctypes.c_uint32),
+                ("max_core", ctypes.c_uint32), ("preprocess_ms", ctypes.c_double)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("algo", ctypes.c_uint32), ("gran", ctypes.c_uint32),
+                ("part_size", ctypes.c_uint32), ("binary", ctypes.c_uint32), ("auto_tune", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("overflow", ctypes.c_uint32), ("algo", ctypes.c_uint32),
+                ("gran", ctypes.c_uint32), ("part_size", ctypes.c_uint32), ("tasks", ctypes.c_uint64),
+                ("alg_bytes", ctypes.c_uint64), ("tree_nodes", ctypes.c_uint64), ("count_ms", ctypes.c_double),
+                ("dominant_ms", ctypes.c_double), ("dominant_bytes", ctypes.c_uint64),
+                ("dominant_class", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32), ("cu_count", ctypes.c_int),
+                ("hbm_bytes", ctypes.c_uint64), ("lds_per_block", ctypes.c_uint64), ("clock_khz", ctypes.c_int)]
+
+
+# every symbol include/kc.h declares, with (restype, argtypes)
+_vp = ctypes.c_void_p
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+SIGNATURES = {
+    "kc_last_error": (ctypes.c_char_p, []),
+    "kc_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.POINTER(_vp)]),
+    "kc_read_edges_reference": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p),
+                                               ctypes.POINTER(ctypes.c_size_t)]),
+    "kc_free_buffer": (None, [_vp]),
+    "kc_graph_from_edges": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_size_t, _u32p, _u32p, ctypes.POINTER(_vp)]),
+    "kc_graph_from_csr": (ctypes.c_int, [ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp)]),
+    "kc_graph_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(GraphInfo)]),
+    "kc_graph_get_csr": (ctypes.c_int, [_vp, _u32p, _u32p]),
+    "kc_graph_free": (None, [_vp]),
+    "kc_generate": (ctypes.c_int, [ctypes.POINTER(GenParams), ctypes.POINTER(_vp)]),
+    "kc_orient": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "kc_oriented_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(OrientedInfo)]),
+    "kc_oriented_export": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
+    "kc_oriented_free": (None, [_vp]),
+    "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
+    "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
+    "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m kcgpu_amd.build` "
+                              "(there is no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(rc):
+    if rc != KC_OK:
+        raise KCError(rc, lib().kc_last_error().decode())
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_u32p)
+
+
+class Graph:
+    """Simple undirected graph (host CSR) -- the reference's loaded COOCSRGraph."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kc_graph_free(self._h)
+            self._h = None
+
+    @classmethod
+    def load(cls, path):
+        h = ctypes.c_void_p()
+        _check(lib().kc_load(os.fsencode(path), 0, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, src, dst, n=0):
+        s, sp = _u32(src)
+        d, dp = _u32(dst)
+        if s.size != d.size:
+            raise ValueError("src and dst differ in length")
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_from_edges(int(n), s.size, sp, dp, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, rowptr, col):
+        r, rp = _u32(rowptr)
+        c, cp = _u32(col)
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_from_csr(r.size - 1, rp, cp, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, kind, seed, n=0, m=0, gamma=0.0, max_degree=0.0, scale=0, edge_factor=0,
+                 a=0.0, b=0.0, c=0.0, attach=0):
+        p = GenParams(kind, n, m, gamma, max_degree, scale, edge_factor, a, b, c, attach, seed)
+        h = ctypes.c_void_p()
+        _check(lib().kc_generate(ctypes.byref(p), ctypes.byref(h)))
+        return cls(h.value)
+
+    def info(self):
+        i = GraphInfo()
+        _check(lib().kc_graph_get_info(self._h, ctypes.byref(i)))
+        return {"n": i.n, "nnz": i.nnz, "m": i.nnz // 2, "max_degree": i.max_degree, "checksum": i.checksum}
+
+    def csr(self):
+        inf = self.info()
+        rp = np.zeros(inf["n"] + 1, dtype=np.uint32)
+        col = np.zeros(max(inf["nnz"], 1), dtype=np.uint32)
+        _check(lib().kc_graph_get_csr(self._h, rp.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p)))
+        return rp, col[: inf["nnz"]]
+
+    def orient(self, mode="degree"):
+        h = ctypes.c_void_p()
+        _check(lib().kc_orient(self._h, ORIENT[mode] if isinstance(mode, str) else int(mode), ctypes.byref(h)))
+        return Oriented(h.value)
+
+
+def read_edges_reference(path):
+    """The reference's raw read loop (defects included): (src, dst) arrays."""
+    s = _u32p()
+    d = _u32p()
+    n = ctypes.c_size_t()
+    _check(lib().kc_read_edges_reference(os.fsencode(path), ctypes.byref(s), ctypes.byref(d), ctypes.byref(n)))
+    try:
+        src = np.ctypeslib.as_array(s, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+        dst = np.ctypeslib.as_array(d, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+    finally:
+        lib().kc_free_buffer(ctypes.cast(s, ctypes.c_void_p))
+        lib().kc_free_buffer(ctypes.cast(d, ctypes.c_void_p))
+    return src, dst
+
+
+def parse_spec(spec):
+    """-q grammar [o|p]<P>[b|n|h] -> dict(algo, part_size, binary)."""
+    q = Query()
+    _check(lib().kc_parse_spec(spec.encode(), ctypes.byref(q)))
+    return {"algo": "pivot" if q.algo == 1 else "orient", "part_size": q.part_size, "binary": bool(q.binary)}
+
+
+@dataclass
+class CountResult:
+    count: int
+    overflow: bool
+    raw: dict
+
+
+class Oriented:
+    """Oriented CSR resident in HBM."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kc_oriented_free(self._h)
+            self._h = None
+
+    def info(self):
+        i = OrientedInfo()
+        _check(lib().kc_oriented_get_info(self._h, ctypes.byref(i)))
+        return {"n": i.n, "m": i.m, "max_out_degree": i.max_out_degree, "max_core": i.max_core,
+                "preprocess_ms": i.preprocess_ms}
+
+    def export(self):
+        inf = self.info()
+        rp = np.zeros(inf["n"] + 1, dtype=np.uint32)
+        col = np.zeros(max(inf["m"], 1), dtype=np.uint32)
+        key = np.zeros(max(inf["n"], 1), dtype=np.uint32)
+        _check(lib().kc_oriented_export(self._h, rp.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p),
+                                        key.ctypes.data_as(_u32p)))
+        return rp, col[: inf["m"]], key[: inf["n"]]
+
+    def count(self, k, algo="pivot", gran="node", part_size=8, binary=True, auto_tune=False, check=True):
+        q = Query(int(k), ALGO[algo] if isinstance(algo, str) else int(algo),
+                  GRAN[gran] if isinstance(gran, str) else int(gran), int(part_size), int(bool(binary)),
+                  int(bool(auto_tune)))
+        r = Result()
+        rc = lib().kc_count(self._h, ctypes.byref(q), ctypes.byref(r))
+        if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
+            _check(rc)
+        return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+def device_info():
+    d = DeviceInfo()
+    _check(lib().kc_get_device_info(ctypes.byref(d)))
+    return {"name": d.name.decode(), "arch": d.arch.decode(), "cu_count": d.cu_count, "hbm_bytes": d.hbm_bytes,
+            "lds_per_block": d.lds_per_block, "clock_khz": d.clock_khz}

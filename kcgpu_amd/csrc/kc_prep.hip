@@ -1,0 +1,598 @@
+// kc_prep.hip -- device preprocessing: exclusive scan, k-core peel (degeneracy
+// key), keep flags, per-row kept counts and stable per-row compaction that
+// turn the symmetric CSR into the oriented CSR.
+//
+// Reference path replaced (src/main.cu:164-235):
+//   getNodeDegree_kernel            kcore/kcore.cuh:50-59
+//   findKcoreIncremental_async      kcore/kcore.cuh:442-584 (host loop, a D->H
+//                                   count read per round) -> one cooperative
+//                                   kernel with grid barriers
+//   init (keep flags)               include/main_support.cuh:54-128
+//   CUBSelect x2                    include/cub_wrappers.cuh:121-179
+//   warp_detect_deleted_edges       include/main_support.cuh:133-167
+//   CUBScanExclusive                include/cub_wrappers.cuh:11-64
+// Here a wave (64 lanes) owns a row: it counts kept entries with a 64-bit
+// ballot, and after the scan writes them in row order at ballot-prefix ranks,
+// which is the stable compaction without a 2m-wide scan.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "kc_device.h"
+
+namespace cg = cooperative_groups;
+
+namespace kc {
+
+// ---------------------------------------------------------------------------
+// device context / buffers
+// ---------------------------------------------------------------------------
+int device_ctx(DeviceCtx **out)
+{
+    static DeviceCtx ctx;
+    static int state = 0;  // 0 untried, 1 ok, 2 failed
+    static std::string why;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (state == 0) {
+        int cnt = 0;
+        hipError_t e = hipGetDeviceCount(&cnt);
+        if (e != hipSuccess || cnt == 0) {
+            state = 2;
+            why = std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "count 0");
+        } else {
+            int d = 0;
+            hipGetDevice(&d);
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, d) != hipSuccess) {
+                state = 2;
+                why = "hipGetDeviceProperties failed";
+            } else if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+                state = 2;
+                why = std::string("device arch ") + p.gcnArchName + " is not gfx950";
+            } else {
+                ctx.dev = d;
+                ctx.prop = p;
+                ctx.cu = p.multiProcessorCount;
+                ctx.lds_per_block = p.sharedMemPerBlock;
+                ctx.hbm = p.totalGlobalMem;
+                state = 1;
+            }
+        }
+    }
+    if (state != 1) return fail(KC_ERR_DEVICE, "%s", why.c_str());
+    hipSetDevice(ctx.dev);
+    *out = &ctx;
+    return KC_OK;
+}
+
+int DevBuf::ensure(size_t want)
+{
+    if (want <= bytes && p) return KC_OK;
+    release();
+    size_t b = want ? want : 256;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        return fail(KC_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    }
+    bytes = b;
+    return KC_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan: tile reduce -> scan of tile sums (one block, carried) ->
+// tile scan with offset.  256 threads x 8 items per tile.
+// ---------------------------------------------------------------------------
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__device__ inline uint64_t wave_incl_scan(uint64_t x)
+{
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(x, (unsigned)d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// block-wide inclusive scan over 256 threads; returns inclusive value, total via *tot
+__device__ inline uint64_t block_incl_scan(uint64_t x, uint64_t *sh, uint64_t *tot)
+{
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t inc = wave_incl_scan(x);
+    if (lane == 63) sh[wid] = inc;
+    __syncthreads();
+    uint64_t off = 0, t = 0;
+    for (int w = 0; w < SCAN_T / 64; ++w) {
+        if (w < wid) off += sh[w];
+        t += sh[w];
+    }
+    __syncthreads();
+    *tot = t;
+    return inc + off;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t *in, uint32_t n, uint64_t *part)
+{
+    __shared__ uint64_t sh[SCAN_T / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t s = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        uint64_t idx = base + (uint64_t)i * SCAN_T + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    uint64_t tot;
+    block_incl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_parts(uint64_t *part, uint32_t np)
+{
+    __shared__ uint64_t sh[SCAN_T / 64];
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < np; b += SCAN_T) {
+        uint32_t i = b + threadIdx.x;
+        uint64_t v = i < np ? part[i] : 0;
+        uint64_t tot;
+        uint64_t inc = block_incl_scan(v, sh, &tot);
+        if (i < np) part[i] = carry + inc - v;  // exclusive
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[np] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_final(const uint32_t *in, uint32_t *out, uint32_t n,
+                                                        const uint64_t *part, uint32_t np)
+{
+    __shared__ uint64_t sh[SCAN_T / 64];
+    // each thread owns SCAN_I consecutive items
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_I;
+    uint32_t v[SCAN_I];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    uint64_t tot;
+    uint64_t inc = block_incl_scan(s, sh, &tot);
+    uint64_t run = part[blockIdx.x] + inc - s;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+        if (base + i < n) out[base + i] = (uint32_t)run;
+        run += v[i];
+    }
+    if (blockIdx.x == np - 1 && threadIdx.x == 0) out[n] = (uint32_t)part[np];
+}
+
+int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tmp, hipStream_t s)
+{
+    if (n == 0) {
+        KC_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+        return KC_OK;
+    }
+    uint32_t np = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int rc = tmp.ensure(sizeof(uint64_t) * (np + 1));
+    if (rc) return rc;
+    uint64_t *part = tmp.as<uint64_t>();
+    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(SCAN_T), 0, s, in, n, part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(SCAN_T), 0, s, part, np);
+    hipLaunchKernelGGL(k_scan_final, dim3(np), dim3(SCAN_T), 0, s, in, out, n, part, np);
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// orientation
+// ---------------------------------------------------------------------------
+// keep(u->v) <=> (key u, u) < (key v, v)   (include/main_support.cuh:54-128)
+__device__ inline bool keep_edge(uint32_t ku, uint32_t u, uint32_t kv, uint32_t v)
+{
+    return ku < kv || (ku == kv && u < v);
+}
+
+template <bool DEGEN>
+__device__ inline uint32_t vkey(const uint32_t *rowptr, const uint32_t *key, uint32_t v)
+{
+    if (DEGEN) return key[v];
+    return rowptr[v + 1] - rowptr[v];
+}
+
+// one wave per row: out-degree after orientation; also records the key
+template <bool DEGEN>
+__global__ __launch_bounds__(256) void k_orient_count(uint32_t n, const uint32_t *rowptr, const uint32_t *col,
+                                                       uint32_t *key, uint32_t *outdeg)
+{
+    int lane = threadIdx.x & 63;
+    uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t u = wave; u < n; u += nw) {
+        uint32_t b = rowptr[u], e = rowptr[u + 1];
+        uint32_t ku = DEGEN ? key[u] : e - b;
+        uint32_t cnt = 0;
+        for (uint32_t i = b; i < e; i += 64) {
+            bool k = false;
+            if (i + lane < e) {
+                uint32_t v = col[i + lane];
+                k = keep_edge(ku, (uint32_t)u, vkey<DEGEN>(rowptr, key, v), v);
+            }
+            cnt += (uint32_t)__popcll(__ballot(k));
+        }
+        if (lane == 0) {
+            outdeg[u] = cnt;
+            if (!DEGEN) key[u] = ku;
+        }
+    }
+}
+
+// one wave per row: stable write of kept entries at ballot-prefix ranks
+template <bool DEGEN>
+__global__ __launch_bounds__(256) void k_orient_write(uint32_t n, const uint32_t *rowptr, const uint32_t *col,
+                                                       const uint32_t *key, const uint32_t *orowptr,
+                                                       uint32_t *ocol, uint32_t *osrc)
+{
+    int lane = threadIdx.x & 63;
+    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t u = wave; u < n; u += nw) {
+        uint32_t b = rowptr[u], e = rowptr[u + 1];
+        uint32_t ku = DEGEN ? key[u] : e - b;
+        uint32_t w = orowptr[u];
+        for (uint32_t i = b; i < e; i += 64) {
+            bool k = false;
+            uint32_t v = 0;
+            if (i + lane < e) {
+                v = col[i + lane];
+                k = keep_edge(ku, (uint32_t)u, vkey<DEGEN>(rowptr, key, v), v);
+            }
+            uint64_t bal = __ballot(k);
+            if (k) {
+                uint32_t r = w + (uint32_t)__popcll(bal & lt);
+                ocol[r] = v;
+                osrc[r] = (uint32_t)u;
+            }
+            w += (uint32_t)__popcll(bal);
+        }
+    }
+}
+
+__global__ void k_max_u32(const uint32_t *a, uint32_t n, uint32_t *out)
+{
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, a[i]);
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// ---------------------------------------------------------------------------
+// k-core peel (kcore/kcore.cuh:442-584) as one cooperative kernel.
+// Level L collects the live vertices of degree L; each round gives its
+// frontier priority = round, then every frontier vertex decrements its live
+// neighbours and a neighbour whose degree was L+1 joins the next round.
+// The round that empties the graph breaks before assigning (kcore.cuh:495-498)
+// so those vertices keep priority = nnz.  All blocks take the same decisions
+// from counters read after a grid barrier.
+// ---------------------------------------------------------------------------
+struct PeelArgs {
+    uint32_t n;
+    uint32_t maxdeg;
+    const uint32_t *rowptr;
+    const uint32_t *col;
+    int *deg;
+    uint32_t *prio;
+    uint8_t *state;  // 0 live, 1 peeled or in frontier
+    uint32_t *q0, *q1;
+    uint32_t *cnt;   // [2] queue counters, [2] max core, [3] error flag
+};
+
+__device__ inline void wave_append(bool take, uint32_t v, uint32_t *q, uint32_t *cnt)
+{
+    int lane = threadIdx.x & 63;
+    uint64_t bal = __ballot(take);
+    if (!bal) return;
+    int leader = __ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (take) q[base + (uint32_t)__popcll(bal & lt)] = v;
+}
+
+__global__ __launch_bounds__(256) void k_peel(PeelArgs a)
+{
+    cg::grid_group grid = cg::this_grid();
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t wave = tid >> 6, nwave = nthr >> 6;
+    const int lane = threadIdx.x & 63;
+    uint32_t *q[2] = {a.q0, a.q1};
+    int c = 0;
+    int64_t todo = a.n;
+    uint32_t level = 0, round = 0;
+    while (todo > 0) {
+        // reset the gather counter, then gather live vertices of degree == level
+        if (tid == 0) a.cnt[c] = 0;
+        grid.sync();
+        for (uint64_t base = wave * 64; base < a.n; base += nwave * 64) {
+            uint64_t v = base + lane;
+            bool take = v < a.n && a.state[v] == 0 && a.deg[v] == (int)level;
+            wave_append(take, (uint32_t)v, q[c], &a.cnt[c]);
+        }
+        grid.sync();
+        uint32_t nc = a.cnt[c];
+        bool final = false;
+        while (nc > 0) {
+            todo -= nc;
+            if (todo == 0) {
+                final = true;
+                break;
+            }
+            if (tid == 0) a.cnt[c ^ 1] = 0;
+            for (uint64_t i = tid; i < nc; i += nthr) {
+                uint32_t v = q[c][i];
+                a.state[v] = 1;
+                a.prio[v] = round;
+            }
+            grid.sync();
+            // decrements: one wave per frontier vertex
+            for (uint64_t i = wave; i < nc; i += nwave) {
+                uint32_t v = q[c][i];
+                uint32_t b = a.rowptr[v], e = a.rowptr[v + 1];
+                for (uint32_t j = b; j < e; j += 64) {
+                    bool take = false;
+                    uint32_t x = 0;
+                    if (j + lane < e) {
+                        x = a.col[j + lane];
+                        if (a.state[x] == 0) {
+                            int old = atomicSub(&a.deg[x], 1);
+                            take = (old == (int)level + 1);
+                        }
+                    }
+                    wave_append(take, x, q[c ^ 1], &a.cnt[c ^ 1]);
+                }
+            }
+            grid.sync();
+            c ^= 1;
+            ++round;
+            nc = a.cnt[c];
+        }
+        if (final) {
+            if (tid == 0) a.cnt[2] = level;
+            break;
+        }
+        // the counter just read (cnt[c] == 0) is reset again by thread 0 at the
+        // top of the next level; make every block finish reading it first
+        grid.sync();
+        ++level;
+        if (level > a.maxdeg + 1) {
+            if (tid == 0) a.cnt[3] = 1;
+            break;
+        }
+    }
+}
+
+__global__ void k_peel_init(uint32_t n, const uint32_t *rowptr, int *deg, uint32_t *prio, uint8_t *state,
+                            uint32_t nnz)
+{
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        deg[v] = (int)(rowptr[v + 1] - rowptr[v]);
+        prio[v] = nnz;
+        state[v] = 0;
+    }
+}
+
+static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uint32_t maxdeg, const uint32_t *rowptr,
+                    const uint32_t *col, uint32_t *prio, uint32_t *max_core)
+{
+    DevBuf deg, st, q0, q1, cnt;
+    int rc;
+    if ((rc = deg.ensure(sizeof(int) * (size_t)n + 4)) || (rc = st.ensure((size_t)n + 4)) ||
+        (rc = q0.ensure(sizeof(uint32_t) * (size_t)n + 4)) || (rc = q1.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
+        (rc = cnt.ensure(sizeof(uint32_t) * 4)))
+        return rc;
+    KC_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * 4, s));
+    int gi = (int)std::min<uint64_t>(((uint64_t)n + 255) / 256, (uint64_t)ctx->cu * 8);
+    if (gi < 1) gi = 1;
+    hipLaunchKernelGGL(k_peel_init, dim3(gi), dim3(256), 0, s, n, rowptr, deg.as<int>(), prio, st.as<uint8_t>(), nnz);
+    KC_HIP(hipGetLastError());
+    int per_cu = 0;
+    KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_peel, 256, 0));
+    if (per_cu < 1) return fail(KC_ERR_DEVICE, "peel kernel cannot be resident");
+    int grid = std::min(per_cu, 4) * ctx->cu;
+    PeelArgs a;
+    a.n = n;
+    a.maxdeg = maxdeg;
+    a.rowptr = rowptr;
+    a.col = col;
+    a.deg = deg.as<int>();
+    a.prio = prio;
+    a.state = st.as<uint8_t>();
+    a.q0 = q0.as<uint32_t>();
+    a.q1 = q1.as<uint32_t>();
+    a.cnt = cnt.as<uint32_t>();
+    void *args[] = {&a};
+    KC_HIP(hipLaunchCooperativeKernel((const void *)k_peel, dim3(grid), dim3(256), args, 0, s));
+    uint32_t h[4];
+    KC_HIP(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+    KC_HIP(hipStreamSynchronize(s));
+    if (h[3]) return fail(KC_ERR_DEVICE, "peel did not converge (internal error)");
+    *max_core = h[2];
+    return KC_OK;
+}
+
+}  // namespace kc
+
+using namespace kc;
+
+kc_oriented::~kc_oriented()
+{
+    if (stream) hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+int kc_get_device_info(kc_device_info *info)
+{
+    if (!info) return fail(KC_ERR_ARG, "kc_get_device_info: null argument");
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    std::memset(info, 0, sizeof *info);
+    std::strncpy(info->name, ctx->prop.name, sizeof info->name - 1);
+    std::strncpy(info->arch, ctx->prop.gcnArchName, sizeof info->arch - 1);
+    info->cu_count = ctx->cu;
+    info->hbm_bytes = ctx->hbm;
+    info->lds_per_block = ctx->lds_per_block;
+    info->clock_khz = ctx->prop.clockRate;
+    return KC_OK;
+}
+
+int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
+{
+    if (!g || !out) return fail(KC_ERR_ARG, "kc_orient: null argument");
+    if (mode != KC_ORIENT_DEGREE && mode != KC_ORIENT_DEGEN) return fail(KC_ERR_ARG, "kc_orient: bad mode %d", mode);
+    *out = nullptr;
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    const uint32_t n = g->csr.n;
+    const uint32_t nnz = (uint32_t)g->csr.col.size();
+    kc_oriented *o = new kc_oriented;
+    o->dev = ctx->dev;
+    o->n = n;
+    o->mode = mode;
+    auto bail = [&](int code) {
+        delete o;
+        return code;
+    };
+    if (hipStreamCreate(&o->stream) != hipSuccess) return bail(fail(KC_ERR_DEVICE, "hipStreamCreate failed"));
+    hipStream_t s = o->stream;
+    DevBuf frow, fcol, outdeg, tmp;
+    if ((rc = frow.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = fcol.ensure(sizeof(uint32_t) * (size_t)nnz + 4)) ||
+        (rc = outdeg.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = o->key.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
+        (rc = o->rowptr.ensure(sizeof(uint32_t) * ((size_t)n + 1))))
+        return bail(rc);
+    if (hipMemcpyAsync(frow.p, g->csr.rowptr.data(), sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        (nnz && hipMemcpyAsync(fcol.p, g->csr.col.data(), sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice, s) !=
+                    hipSuccess))
+        return bail(fail(KC_ERR_DEVICE, "graph upload failed"));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, s);
+    uint32_t max_core = 0;
+    if (mode == KC_ORIENT_DEGEN && n > 0) {
+        rc = run_peel(ctx, s, n, nnz, g->max_degree, frow.as<uint32_t>(), fcol.as<uint32_t>(), o->key.as<uint32_t>(),
+                      &max_core);
+        if (rc) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            return bail(rc);
+        }
+    }
+    int grid = (int)std::min<uint64_t>(((uint64_t)n + 3) / 4, (uint64_t)ctx->cu * 16);
+    if (grid < 1) grid = 1;
+    if (n > 0) {
+        if (mode == KC_ORIENT_DEGEN)
+            hipLaunchKernelGGL(k_orient_count<true>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
+                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), outdeg.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_orient_count<false>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
+                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), outdeg.as<uint32_t>());
+    }
+    if ((rc = scan_exclusive_u32(outdeg.as<uint32_t>(), o->rowptr.as<uint32_t>(), n, tmp, s))) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return bail(rc);
+    }
+    uint32_t m = 0;
+    hipMemcpyAsync(&m, o->rowptr.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return bail(fail(KC_ERR_DEVICE, "orientation failed: %s", hipGetErrorString(hipGetLastError())));
+    }
+    o->m = m;
+    if ((rc = o->col.ensure(sizeof(uint32_t) * (size_t)m + 4)) || (rc = o->src.ensure(sizeof(uint32_t) * (size_t)m + 4))) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return bail(rc);
+    }
+    if (n > 0) {
+        if (mode == KC_ORIENT_DEGEN)
+            hipLaunchKernelGGL(k_orient_write<true>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
+                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), o->rowptr.as<uint32_t>(),
+                               o->col.as<uint32_t>(), o->src.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(k_orient_write<false>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
+                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), o->rowptr.as<uint32_t>(),
+                               o->col.as<uint32_t>(), o->src.as<uint32_t>());
+    }
+    // max out-degree
+    DevBuf mx;
+    if ((rc = mx.ensure(sizeof(uint32_t)))) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return bail(rc);
+    }
+    hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s);
+    if (n > 0)
+        hipLaunchKernelGGL(k_max_u32, dim3(std::max(1, std::min(grid, 1024))), dim3(256), 0, s, outdeg.as<uint32_t>(),
+                           n, mx.as<uint32_t>());
+    hipEventRecord(e1, s);
+    uint32_t maxout = 0;
+    hipMemcpyAsync(&maxout, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    hipError_t se = hipStreamSynchronize(s);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (se != hipSuccess) return bail(fail(KC_ERR_DEVICE, "orientation failed: %s", hipGetErrorString(se)));
+    o->max_out = maxout;
+    o->max_core = max_core;
+    o->pre_ms = ms;
+    *out = o;
+    return KC_OK;
+}
+
+int kc_oriented_get_info(const kc_oriented *o, kc_oriented_info *info)
+{
+    if (!o || !info) return fail(KC_ERR_ARG, "kc_oriented_get_info: null argument");
+    info->n = o->n;
+    info->m = o->m;
+    info->max_out_degree = o->max_out;
+    info->max_core = o->max_core;
+    info->preprocess_ms = o->pre_ms;
+    return KC_OK;
+}
+
+int kc_oriented_export(const kc_oriented *o, uint32_t *rowptr, uint32_t *col, uint32_t *key)
+{
+    if (!o) return fail(KC_ERR_ARG, "kc_oriented_export: null handle");
+    hipSetDevice(o->dev);
+    if (rowptr) KC_HIP(hipMemcpy(rowptr, o->rowptr.p, sizeof(uint32_t) * ((size_t)o->n + 1), hipMemcpyDeviceToHost));
+    if (col && o->m) KC_HIP(hipMemcpy(col, o->col.p, sizeof(uint32_t) * (size_t)o->m, hipMemcpyDeviceToHost));
+    if (key && o->n) KC_HIP(hipMemcpy(key, o->key.p, sizeof(uint32_t) * (size_t)o->n, hipMemcpyDeviceToHost));
+    return KC_OK;
+}
+
+void kc_oriented_free(kc_oriented *o) { delete o; }
+
+}  // extern "C"

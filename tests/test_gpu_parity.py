@@ -1,0 +1,180 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the golden counts.
+
+All tests need an MI355X; they call libkcgpu.so (kcgpu_amd) and compare bit
+for bit (integer counts, tolerance 0).
+"""
+from math import comb
+
+import numpy as np
+import pytest
+
+import kcgpu_amd as K
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRANS = ("node", "edge")
+ALGOS = ("orient", "pivot")
+
+
+def _graph(g):
+    e = np.array(g["edges"], dtype=np.uint32).reshape(-1, 2)
+    return K.Graph.from_edges(e[:, 0], e[:, 1], g["n"])
+
+
+def _host_csr(G):
+    rp, col = G.csr()
+    return rp.astype(np.uint64), col
+
+
+def test_device_is_gfx950():
+    d = K.device_info()
+    print(d, flush=True)
+    assert d["arch"].startswith("gfx950")
+    assert d["cu_count"] > 0
+
+
+@pytest.mark.parametrize("mode", ["degree", "degen"])
+def test_orientation_matches_oracle(golden, mode):
+    for g in golden:
+        G = _graph(g)
+        rp, col = _host_csr(G)
+        Og = G.orient(mode)
+        orp, oc, key = Og.export()
+        if mode == "degen":
+            pr, mc = O.peel_priority(rp, col)
+            assert (key == pr).all(), g["name"]
+            assert Og.info()["max_core"] == mc == g["max_core"], g["name"]
+            erp, ec = O.orient(rp, col, pr)
+        else:
+            assert (key == np.diff(rp)).all(), g["name"]
+            erp, ec = O.orient(rp, col)
+        assert (orp == erp).all() and (oc == ec).all(), g["name"]
+        assert Og.info()["max_out_degree"] == int(np.diff(erp).max(initial=0))
+
+
+@pytest.mark.parametrize("mode", ["degree", "degen"])
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("gran", GRANS)
+def test_counts_vs_golden(golden, mode, algo, gran):
+    for g in golden:
+        Og = _graph(g).orient(mode)
+        for k in range(1, 11):
+            if algo == "orient" and g["name"] == "K25" and k > 8:
+                continue
+            for P in (1, 8, 64):
+                r = Og.count(k, algo, gran, part_size=P)
+                assert r.count == g["counts"][k] and not r.overflow, (g["name"], mode, algo, gran, k, P)
+
+
+def test_partition_sizes_all_agree(golden):
+    g = next(x for x in golden if x["name"] == "planted_400")
+    Og = _graph(g).orient("degen")
+    for algo in ALGOS:
+        for gran in GRANS:
+            for P in (1, 2, 4, 8, 16, 32, 64):
+                for k in (4, 7):
+                    r = Og.count(k, algo, gran, part_size=P)
+                    assert r.count == g["counts"][k], (algo, gran, P, k)
+
+
+def _planted(n, m, sizes, seed):
+    """sparse ER with planted cliques: out-degrees reach every size class"""
+    rng = np.random.default_rng(seed)
+    G0 = K.Graph.generate(K.GEN_ER, seed=seed, n=n, m=m)
+    rp, col = G0.csr()
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    es, ed = [src], [col]
+    for s in sizes:
+        vs = rng.choice(n, size=s, replace=False).astype(np.uint32)
+        a, b = np.triu_indices(s, 1)
+        es.append(vs[a])
+        ed.append(vs[b])
+    return K.Graph.from_edges(np.concatenate(es), np.concatenate(ed), n)
+
+
+@pytest.mark.parametrize("mode", ["degree", "degen"])
+def test_size_classes_vs_oracle(mode):
+    # out-degrees above 64, 128, 256, 512 and 1024 exercise every LDS class and the HBM class
+    G = _planted(6000, 30000, [80, 150, 300, 600, 1100], seed=5)
+    rp, col = _host_csr(G)
+    key = O.peel_priority(rp, col)[0] if mode == "degen" else None
+    orp, oc = O.orient(rp, col, key)
+    Og = G.orient(mode)
+    assert Og.info()["max_out_degree"] > 1024
+    for k, algos in ((3, ALGOS), (4, ALGOS), (7, ("pivot",))):
+        exp = O.count(orp, oc, k, "pivot", "node")["count"]
+        for algo in algos:
+            for gran in GRANS:
+                r = Og.count(k, algo, gran, part_size=8)
+                print(mode, k, algo, gran, r.count, r.raw["count_ms"], flush=True)
+                assert r.count == exp, (mode, k, algo, gran)
+
+
+@pytest.mark.parametrize("kind,kw", [
+    (K.GEN_ER, dict(n=10000, m=100000)),  # config 1 shape
+    (K.GEN_BA, dict(n=20000, attach=12)),
+    (K.GEN_CHUNG_LU, dict(n=30000, m=200000, gamma=2.1, max_degree=1500)),
+    (K.GEN_RMAT, dict(scale=13, edge_factor=16, a=0.57, b=0.19, c=0.19)),
+])
+def test_generated_graphs_vs_oracle(kind, kw):
+    G = K.Graph.generate(kind, seed=11, **kw)
+    rp, col = _host_csr(G)
+    pr, _ = O.peel_priority(rp, col)
+    for mode, key in (("degree", None), ("degen", pr)):
+        orp, oc = O.orient(rp, col, key)
+        Og = G.orient(mode)
+        xr, xc, _ = Og.export()
+        assert (xr == orp).all() and (xc == oc).all()
+        for k in (3, 4, 5, 7):
+            exp = O.count(orp, oc, k, "orient", "edge")
+            for algo in ALGOS:
+                for gran in GRANS:
+                    r = Og.count(k, algo, gran, part_size=16)
+                    assert r.count == exp["count"], (kind, mode, k, algo, gran)
+                    if algo == "orient" and gran == "edge":
+                        assert r.raw["tasks"] == exp["tasks"]
+
+
+def test_overflow_flag_and_exact_boundary():
+    n = 70
+    a, b = np.triu_indices(n, 1)
+    Og = K.Graph.from_edges(a.astype(np.uint32), b.astype(np.uint32), n).orient("degree")
+    for gran in GRANS:
+        r = Og.count(30, "pivot", gran, check=False)
+        assert r.overflow
+        with pytest.raises(K.KCError) as ei:
+            Og.count(30, "pivot", gran)
+        assert ei.value.code == K.KC_ERR_OVERFLOW
+        r = Og.count(32, "pivot", gran, check=False)
+        assert r.overflow
+        r = Og.count(12, "pivot", gran)
+        assert r.count == comb(70, 12) and not r.overflow
+    n = 66  # C(66, 32) = 7.0e18 < 2^64: exact
+    a, b = np.triu_indices(n, 1)
+    Og = K.Graph.from_edges(a.astype(np.uint32), b.astype(np.uint32), n).orient("degen")
+    for gran in GRANS:
+        r = Og.count(32, "pivot", gran)
+        assert r.count == comb(66, 32) and not r.overflow
+
+
+def test_bad_queries_are_errors():
+    Og = K.Graph.generate(K.GEN_ER, seed=1, n=100, m=300).orient("degree")
+    for bad in (dict(k=0), dict(k=33), dict(k=4, part_size=3), dict(k=4, part_size=128)):
+        kw = dict(algo="pivot", gran="node")
+        kw.update(bad)
+        with pytest.raises(K.KCError) as ei:
+            Og.count(**kw)
+        assert ei.value.code == K.KC_ERR_ARG
+
+
+def test_empty_and_tiny_graphs():
+    for n, src, dst in ((1, [], []), (2, [0], [1]), (3, [0, 1], [1, 2])):
+        G = K.Graph.from_edges(np.array(src, np.uint32), np.array(dst, np.uint32), n)
+        for mode in ("degree", "degen"):
+            Og = G.orient(mode)
+            for k in range(1, 5):
+                exp = {1: n, 2: len(src)}.get(k, 0)
+                for algo in ALGOS:
+                    for gran in GRANS:
+                        assert Og.count(k, algo, gran).count == exp
