@@ -168,6 +168,14 @@ typedef struct {
     double dominant_ms;   /* device time of the longest counting kernel */
     uint64_t dominant_bytes; /* algorithmic bytes of that kernel's tasks */
     uint32_t dominant_class; /* its size class (0: <=64, 1: <=128, 2: <=256, 3: <=512, 4: <=1024, 5: larger) */
+    /* per size class: device time, algorithmic bytes, tasks, kernel variant
+     * (0 none, 1 LDS bitmap, 2 register bitmap, 3 HBM bitmap) */
+    double class_ms[6];
+    uint64_t class_bytes[6];
+    uint64_t class_tasks[6];
+    uint32_t class_kernel[6];
+    double class_max_task_ms[6];  /* longest single task (wave wall clock) */
+    double class_task_ms_sum[6];  /* sum of task durations over all waves */
 } kc_result;
 
 /*
@@ -199,6 +207,10 @@ typedef struct {
 } kc_device_info;
 
 int kc_get_device_info(kc_device_info *info);
+
+/* Select the HIP device (the reference's -d, include/Config.h:345) for
+ * handles created after this call.  The device must be a gfx950. */
+int kc_select_device(int ordinal);
 
 #ifdef __cplusplus
 }

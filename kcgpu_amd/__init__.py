@@ -69,10 +69,17 @@ class Result(ctypes.Structure):
                 ("gran", ctypes.c_uint32), ("part_size", ctypes.c_uint32), ("tasks", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_uint64), ("tree_nodes", ctypes.c_uint64), ("count_ms", ctypes.c_double),
                 ("dominant_ms", ctypes.c_double), ("dominant_bytes", ctypes.c_uint64),
-                ("dominant_class", ctypes.c_uint32)]
+                ("dominant_class", ctypes.c_uint32), ("class_ms", ctypes.c_double * 6),
+                ("class_bytes", ctypes.c_uint64 * 6), ("class_tasks", ctypes.c_uint64 * 6),
+                ("class_kernel", ctypes.c_uint32 * 6), ("class_max_task_ms", ctypes.c_double * 6),
+                ("class_task_ms_sum", ctypes.c_double * 6)]
 
     def as_dict(self):
-        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+        d = {}
+        for f in self._fields_:
+            v = getattr(self, f[0])
+            d[f[0]] = list(v) if hasattr(v, "_length_") else v
+        return d
 
 
 class DeviceInfo(ctypes.Structure):
@@ -102,6 +109,7 @@ SIGNATURES = {
     "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
+    "kc_select_device": (ctypes.c_int, [ctypes.c_int]),
 }
 
 _lib = None

@@ -1,0 +1,172 @@
+// kc_main.cpp -- drop-in command line for the reference's `-m kc` path
+// (src/main.cu:35-588, flags from include/Config.h:317-363).
+//
+//   kcgpu -g graph.{mtx,tsv,txt,bel} [-m kc|kcore] [-o degree|degen] [-k K]
+//         [-p node|edge] [-q {o|p}<P>[b|n]] [-d dev] [-w x] [-s x] [-x x] [-a ...]
+//
+// Every reference flag takes an argument (getopt "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:");
+// -w, -s, -x, -a, -e, -v, -r, -t are accepted for compatibility and do not
+// change the result.  Output keeps the reference's lines (Read graph time,
+// Preprocess time, Max Core, the Level/Nodes/Counter line, count time and the
+// MOHA ... teps line) with the true count in place of the reference's level
+// value, plus a `kcliques=` line and one JSON metrics line.
+#include <getopt.h>
+
+#include <cinttypes>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+
+#include "../../include/kc.h"
+
+namespace {
+
+void stamp(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void stamp(const char *fmt, ...)
+{
+    std::time_t t = std::time(nullptr);
+    char ts[16];
+    std::strftime(ts, sizeof ts, "%H:%M:%S", std::localtime(&t));
+    std::printf("[%s] ", ts);
+    va_list ap;
+    va_start(ap, fmt);
+    std::vprintf(fmt, ap);
+    va_end(ap);
+    std::printf("\n");
+}
+
+double now()
+{
+    timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+void usage()
+{
+    std::fprintf(stderr,
+                 "\nUsage:  kcgpu [options]\n"
+                 "    -g <graph file>   .mtx, .tsv, .txt or .bel\n"
+                 "    -m <task>         kc (k-clique count, default) or kcore (max core)\n"
+                 "    -o <orient>       degree (default) or degen\n"
+                 "    -k <k>            clique size (default 5)\n"
+                 "    -p <unit>         node (default) or edge\n"
+                 "    -q <spec>         [o|p]<P>[b|n]: orientation or pivoting, partition size\n"
+                 "                      P in 1,2,4,...,64, bitmap (b) or index lists (n); default o8b\n"
+                 "    -d <device>       HIP device ordinal (default 0)\n"
+                 "    -t auto           let the autotuner pick algorithm, granularity and P\n"
+                 "    -w/-s/-x/-a/-e/-v/-r <arg>  accepted for compatibility\n"
+                 "    -h <any>          help\n\n");
+}
+
+int die(int rc, const char *what)
+{
+    std::fprintf(stderr, "error: %s: %s (status %d)\n", what, kc_last_error(), rc);
+    return rc == 0 ? 1 : rc;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const char *graph = nullptr;
+    std::string task = "kc", orient = "degree", by = "node", spec = "o8b";
+    int k = 5, dev = 0;
+    bool autotune = false;
+    int opt;
+    while ((opt = getopt(argc, argv, "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:")) >= 0) {
+        switch (opt) {
+        case 'g': graph = optarg; break;
+        case 'd': dev = std::atoi(optarg); break;
+        case 'm': task = optarg; break;
+        case 'o': orient = optarg; break;
+        case 'k': k = std::atoi(optarg); break;
+        case 'p': by = optarg; break;
+        case 'q': spec = optarg; break;
+        case 't': autotune = std::strcmp(optarg, "auto") == 0; break;
+        case 'r': case 'x': case 'a': case 'v': case 's': case 'e': case 'w': break;
+        case 'h': usage(); return 0;
+        default: usage(); return 1;
+        }
+    }
+    if (!graph) {
+        usage();
+        return 1;
+    }
+    if (task != "kc" && task != "kcore") {
+        std::fprintf(stderr, "error: -m %s is not part of this build (kc, kcore)\n", task.c_str());
+        return 1;
+    }
+    int omode;
+    if (orient == "degree") omode = KC_ORIENT_DEGREE;
+    else if (orient == "degen") omode = KC_ORIENT_DEGEN;
+    else {
+        std::fprintf(stderr, "error: -o %s: this build orients by degree or degen\n", orient.c_str());
+        return 1;
+    }
+    if (task == "kcore") omode = KC_ORIENT_DEGEN;
+    kc_query q{};
+    q.k = (uint32_t)k;
+    q.gran = by == "edge" ? KC_GRAN_EDGE : KC_GRAN_NODE;
+    if (by != "edge" && by != "node") {
+        std::fprintf(stderr, "error: -p %s: node or edge\n", by.c_str());
+        return 1;
+    }
+    int rc = kc_parse_spec(spec.c_str(), &q);
+    if (rc) return die(rc, "-q");
+    q.auto_tune = autotune ? 1 : 0;
+
+    std::printf("Welcome ---------------------\n");
+    std::printf("    Graph: %s\n    Device Id = %d\n    Main Task = %s\n    Graph Orientation = %s\n"
+                "    Process By = %s\n    k: %d\n    KC Config = %s-%u-%s\n",
+                graph, dev, task.c_str(), orient.c_str(), by.c_str(), k, q.algo ? "PIV" : "GO", q.part_size,
+                q.binary ? "BE" : "NBE");
+    if ((rc = kc_select_device(dev))) return die(rc, "device");
+
+    double t0 = now();
+    kc_graph *g = nullptr;
+    if ((rc = kc_load(graph, 0, &g))) return die(rc, "load");
+    kc_graph_info gi;
+    kc_graph_get_info(g, &gi);
+    stamp("Read graph time: %f s", now() - t0);
+    std::printf("    n = %u, undirected edges = %" PRIu64 ", max degree = %u\n", gi.n, gi.nnz / 2, gi.max_degree);
+
+    double t1 = now();
+    kc_oriented *o = nullptr;
+    if ((rc = kc_orient(g, omode, &o))) return die(rc, "orient");
+    kc_oriented_info oi;
+    kc_oriented_get_info(o, &oi);
+    if (omode == KC_ORIENT_DEGEN) std::printf("Max Core = %u\n", oi.max_core);
+    stamp("Preprocess time: %f s", now() - t1);
+    if (task == "kcore") {
+        kc_oriented_free(o);
+        kc_graph_free(g);
+        return 0;
+    }
+
+    kc_result r;
+    double t2 = now();
+    rc = kc_count(o, &q, &r);
+    double tc = now() - t2;
+    if (rc && rc != KC_ERR_OVERFLOW) return die(rc, "count");
+    const unsigned level = (r.gran == KC_GRAN_EDGE) ? (k >= 2 ? k - 2 : 0) : (k >= 1 ? k - 1 : 0);
+    if (r.gran == KC_GRAN_NODE)
+        std::printf("------------- Level = %u Nodes = %" PRIu64 " Counter = %" PRIu64 "\n", level, r.tasks, r.count);
+    else
+        std::printf("Level = %u Nodes = %" PRIu64 " Counter = %" PRIu64 "\n", level, r.tasks, r.count);
+    stamp("count time %f s", tc);
+    stamp("MOHA %" PRIu64 " k-clique (%f teps)", r.count, tc > 0 ? (double)oi.m / tc : 0.0);
+    std::printf("kcliques=%" PRIu64 " k=%d overflow=%u\n", r.count, k, r.overflow);
+    std::printf("{\"k\": %d, \"count\": %" PRIu64 ", \"overflow\": %u, \"algo\": \"%s\", \"gran\": \"%s\", \"part_size\": %u, "
+                "\"n\": %u, \"m_oriented\": %" PRIu64 ", \"preprocess_ms\": %.3f, \"count_ms\": %.3f, "
+                "\"kcliques_per_s\": %.6g, \"edges_per_s\": %.6g, \"tasks\": %" PRIu64 "}\n",
+                k, r.count, r.overflow, r.algo ? "pivot" : "orient", r.gran ? "edge" : "node", r.part_size, oi.n,
+                oi.m, oi.preprocess_ms, r.count_ms, r.count_ms > 0 ? r.count / (r.count_ms / 1e3) : 0.0,
+                r.count_ms > 0 ? oi.m / (r.count_ms / 1e3) : 0.0, r.tasks);
+    kc_oriented_free(o);
+    kc_graph_free(g);
+    return r.overflow ? KC_ERR_OVERFLOW : 0;
+}

@@ -56,7 +56,7 @@ struct kc_oriented {
     double pre_ms = 0;
     kc::DevBuf rowptr, col, src, key;
     // count scratch (grow-only, reused across calls on this handle)
-    kc::DevBuf s_tasks, s_meta, s_binom, s_gscr, s_scan;
+    kc::DevBuf s_tasks, s_meta, s_binom, s_gscr, s_scan, s_tsz;
     std::vector<uint32_t> h_rowptr_cache;  // filled lazily by host-side autotune
     ~kc_oriented();
 };

@@ -30,13 +30,17 @@ namespace kc {
 // ---------------------------------------------------------------------------
 // device context / buffers
 // ---------------------------------------------------------------------------
+static DeviceCtx g_ctx;
+static int g_state = 0;  // 0 untried, 1 ok, 2 failed
+static std::string g_why;
+static std::mutex g_mu;
+
 int device_ctx(DeviceCtx **out)
 {
-    static DeviceCtx ctx;
-    static int state = 0;  // 0 untried, 1 ok, 2 failed
-    static std::string why;
-    static std::mutex mu;
-    std::lock_guard<std::mutex> lk(mu);
+    DeviceCtx &ctx = g_ctx;
+    int &state = g_state;
+    std::string &why = g_why;
+    std::lock_guard<std::mutex> lk(g_mu);
     if (state == 0) {
         int cnt = 0;
         hipError_t e = hipGetDeviceCount(&cnt);
@@ -461,6 +465,20 @@ int kc_get_device_info(kc_device_info *info)
     info->lds_per_block = ctx->lds_per_block;
     info->clock_khz = ctx->prop.clockRate;
     return KC_OK;
+}
+
+int kc_select_device(int ordinal)
+{
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return fail(KC_ERR_DEVICE, "no HIP device");
+    if (ordinal < 0 || ordinal >= cnt) return fail(KC_ERR_ARG, "device %d outside 0..%d", ordinal, cnt - 1);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (hipSetDevice(ordinal) != hipSuccess) return fail(KC_ERR_DEVICE, "hipSetDevice(%d) failed", ordinal);
+        g_state = 0;  // re-read the properties of the selected device
+    }
+    DeviceCtx *ctx;
+    return device_ctx(&ctx);
 }
 
 int kc_orient(const kc_graph *g, int mode, kc_oriented **out)

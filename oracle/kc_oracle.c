@@ -187,6 +187,7 @@ typedef struct {
     u32 *list;      /* task vertex list (N+(u) or T) */
     u32 k;
     int ovf;        /* a needed binomial did not fit in 64 bits */
+    int tails;      /* close SCT subtrees with held >= k-2 in closed form */
     u128 bytes;     /* algorithmic bytes of the processed tasks */
 } ctx_t;
 
@@ -273,10 +274,18 @@ static u128 sct(ctx_t *c, const u64 *C, u32 held, u32 piv, u32 depth)
         if (o) c->ovf = 1;
         return v;
     }
+    u32 cs = (u32)popc_set(C, W);
+    /* closed-form tails (not in the reference): a subtree with h held and
+     * p pivot vertices over candidates C holds sum_j C(p, k-h-j) c_j(C)
+     * k-cliques, c_j = j-cliques of G[C]; c_0 = 1, c_1 = |C|, c_2 and c_3
+     * below.  Tails close every node with h >= k-3. */
+    if (c->tails && held == k) return 1;
+    if (c->tails && held + 1 == k) return (u128)piv + cs;
     /* a clique below this node has at most held + piv + |C| vertices */
-    if (held + piv + (u32)popc_set(C, W) < k) return 0;
+    if (held + piv + cs < k) return 0;
     /* pivot choice */
     int best = -1; u32 bj = 0;
+    u128 score_sum = 0;
     for (u32 wi = 0; wi < W; ++wi) {
         u64 bits = C[wi];
         while (bits) {
@@ -285,8 +294,40 @@ static u128 sct(ctx_t *c, const u64 *C, u32 held, u32 piv, u32 depth)
             const u64 *rj = c->rows + (size_t)j * W;
             int cnt = 0;
             for (u32 w = 0; w < W; ++w) cnt += popc64(C[w] & rj[w]);
+            score_sum += (u128)cnt;
             if (cnt > best) { best = cnt; bj = j; }
         }
+    }
+    /* c_2 = |E(C)| = half the sum of the scores */
+    if (c->tails && held + 2 == k)
+        return binom128(piv, 2) + (u128)piv * cs + score_sum / 2;
+    /* c_3 = triangles i < j < l of G[C], each counted at its smallest vertex */
+    if (c->tails && held + 3 == k) {
+        u128 tri = 0;
+        for (u32 wi = 0; wi < W; ++wi) {
+            u64 bi = C[wi];
+            while (bi) {
+                u32 i = wi * 64 + (u32)__builtin_ctzll(bi);
+                bi &= bi - 1;
+                const u64 *ri = c->rows + (size_t)i * W;
+                for (u32 wj = 0; wj < W; ++wj) {
+                    u64 bj2 = C[wj] & ri[wj];
+                    if (wj < (i >> 6)) continue;
+                    if (wj == (i >> 6)) bj2 &= (i & 63) == 63 ? 0 : (~0ull << ((i & 63) + 1));
+                    while (bj2) {
+                        u32 j = wj * 64 + (u32)__builtin_ctzll(bj2);
+                        bj2 &= bj2 - 1;
+                        const u64 *rj = c->rows + (size_t)j * W;
+                        for (u32 w = j >> 6; w < W; ++w) {
+                            u64 x = C[w] & ri[w] & rj[w];
+                            if (w == (j >> 6)) x &= (j & 63) == 63 ? 0 : (~0ull << ((j & 63) + 1));
+                            tri += (u128)popc64(x);
+                        }
+                    }
+                }
+            }
+        }
+        return binom128(piv, 3) + binom128(piv, 2) * cs + (u128)piv * (score_sum / 2) + tri;
     }
     /* ... and at most held + piv + 1 + max_j |C & row_j| (the largest
      * clique inside C has at most maxIntersection + 1 vertices; the edge
@@ -329,15 +370,16 @@ static u32 intersect(const u32 *a, u64 la, const u32 *b, u64 lb, u32 *out)
 /*
  * orc_count: number of k-cliques of the graph whose oriented CSR is
  * (rowptr, col), by one of the reference's four traversals.
- *   algo: 0 = orientation DFS, 1 = pivoting (SCT)
+ *   algo: 0 = orientation DFS, 1 = pivoting (SCT), 2 = pivoting with the
+ *         closed-form tails used by the HIP kernels (same counts)
  *   gran: 0 = node tasks, 1 = edge tasks
  * Tasks are numbered 0..n-1 (node) or 0..m-1 (edge, in CSR order); only the
  * tasks with index % stride == offset are processed (stride=1: all), which
  * gives the bounded CPU-baseline sample.  Task filters: node tasks need
  * d+(u) >= k-1 (kclique/kclique.cuh:1610-1614); edge tasks need
- * min(d+(u), d+(v)) >= k-2 (the reference filters on the max,
- * kclique/kclique.cuh:2344-2355 with common.cuh:18-46) and are counted as
- * done; their rows are only built when |T| >= k-2.
+ * |T| >= k-2 where T = N+(u) & N+(v) (the reference filters on
+ * max(d+(u), d+(v)), kclique/kclique.cuh:2344-2355 with common.cuh:18-46,
+ * which keeps tasks that cannot hold a clique).
  * nthreads <= 0 uses the OpenMP default.
  * Returns 0; *count gets the low 64 bits, *overflow is set when the count or
  * a binomial it needs is >= 2^64.  *tasks_done gets the number of tasks that
@@ -402,6 +444,7 @@ int orc_count(u32 n, const u64 *rowptr, const u32 *col, u32 k, int algo, int gra
         c.stack = (u64 *)malloc(sizeof(u64) * ((size_t)maxd + k + 4) * 3 * Wmax);
         c.list = (u32 *)malloc(sizeof(u32) * ((size_t)maxd + 1));
         c.ovf = 0;
+        c.tails = (algo == 2);
         c.bytes = 0;
         u128 acc = 0;
         u64 done = 0;
@@ -422,15 +465,17 @@ int orc_count(u32 n, const u64 *rowptr, const u32 *col, u32 k, int algo, int gra
                 u32 u = src_of[t], v = col[t];
                 u64 du = rowptr[u + 1] - rowptr[u], dv = rowptr[v + 1] - rowptr[v];
                 if (du < k - 2 || dv < k - 2) continue;
-                done++;
-                c.bytes += 24 + 4 * (u128)(du + dv);
                 R = intersect(col + rowptr[u], du, col + rowptr[v], dv, c.list);
                 if (R < k - 2) continue;
+                c.bytes += 24 + 4 * (u128)(du + dv);
+                /* k=3 orientation adds |T| without a bitmap
+                 * (kckernels_orientation.cuh:546-547) */
+                if (algo == 0 && k == 3) { done++; acc += R; continue; }
             }
-            if (gran == 0) done++;
+            done++;
             c.W = (R + 63) / 64;
             if (c.W == 0) c.W = 1;
-            encode(&c, rowptr, col, R, algo == 1);
+            encode(&c, rowptr, col, R, algo != 0);
             if (algo == 0) {
                 u32 need = (gran == 0) ? k - 1 : k - 2;   /* vertices still to pick (>= 1) */
                 if (gran == 0) {
@@ -441,9 +486,7 @@ int orc_count(u32 n, const u64 *rowptr, const u32 *col, u32 k, int algo, int gra
                             acc += dag_count(&c, rj, need - 1, 0);
                     }
                 } else {
-                    /* k=3 adds |T| (kckernels_orientation.cuh:546-547);
-                     * deeper levels start from each row of T (:562-633) */
-                    if (need == 1) { acc += R; continue; }
+                    /* deeper levels start from each row of T (:562-633) */
                     for (u32 j = 0; j < R; ++j) {
                         const u64 *rj = c.rows + (size_t)j * c.W;
                         if ((u32)popc_set(rj, c.W) >= need - 1)

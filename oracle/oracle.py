@@ -135,7 +135,7 @@ def peel_priority(rowptr, col):
     return prio[:n], int(mc)
 
 
-ALGO = {"orient": 0, "pivot": 1}
+ALGO = {"orient": 0, "pivot": 1, "pivot_tails": 2}
 GRAN = {"node": 0, "edge": 1}
 
 

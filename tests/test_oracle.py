@@ -113,7 +113,7 @@ def test_binom_exact_and_overflow():
 
 
 # ---- counting -----------------------------------------------------------------------
-@pytest.mark.parametrize("algo", ["orient", "pivot"])
+@pytest.mark.parametrize("algo", ["orient", "pivot", "pivot_tails"])
 @pytest.mark.parametrize("gran", ["node", "edge"])
 def test_counts_vs_golden(golden, algo, gran):
     for g in golden:
@@ -160,7 +160,7 @@ def test_sampled_tasks_partition_the_count(golden):
     g = next(x for x in golden if x["name"] == "planted_400")
     rp, col = _csr(g)
     orp, oc = O.orient(rp, col)
-    for algo in ("orient", "pivot"):
+    for algo in ("orient", "pivot", "pivot_tails"):
         for gran in ("node", "edge"):
             parts = [O.count(orp, oc, 6, algo, gran, stride=3, offset=o)["count"] for o in range(3)]
             assert sum(parts) == g["counts"][6]
