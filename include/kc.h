@@ -196,6 +196,81 @@ int kc_count(const kc_oriented *o, const kc_query *q, kc_result *result);
  */
 int kc_parse_spec(const char *spec, kc_query *q);
 
+/* ---- autotuner ----------------------------------------------------------- */
+typedef struct {
+    int orient_mode;        /* KC_ORIENT_* to pass to kc_orient */
+    uint32_t algo, gran, part_size;
+    uint32_t degree_max_out;  /* max out-degree the degree orientation would give */
+} kc_plan;
+
+/*
+ * Pick orientation, granularity, algorithm and partition size for counting
+ * k-cliques of g (the north-star autotuner; the reference takes all four
+ * from the command line).  One O(m) host pass over the CSR.
+ */
+int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan);
+
+/* ---- batches: many (graph, query) pairs in one persistent launch ------- */
+typedef struct {
+    double total_ms;    /* device time: task sizing, binning and counting */
+    double count_ms;    /* device time of the counting launches */
+    uint64_t tasks;     /* tasks drained from the device queues */
+    uint64_t queries;   /* queries that needed the GPU (k >= 3) */
+} kc_batch_stats;
+
+/*
+ * Count nq queries, query i on graph gs[i] (graphs may repeat), in
+ * persistent launches that drain one device queue of (query, vertex|edge)
+ * tasks per (algorithm, size class) bin -- for small graphs a single launch.
+ * results[i] gets exactly the count kc_count returns for query i alone.
+ * All graphs must live on one device; the call uses the first graph's stream.
+ * No counterpart in the reference (it counts one graph per process).
+ */
+int kc_batch_count(kc_oriented *const *gs, const kc_query *qs, size_t nq, kc_result *results,
+                   kc_batch_stats *stats);
+
+/* ---- serving ------------------------------------------------------------ */
+typedef struct kc_server kc_server;
+
+typedef struct {
+    uint64_t budget_bytes;  /* HBM for resident CSRs + launch scratch; 0 = 90% of free memory */
+    uint32_t workers;       /* batches in flight at once (one stream each); 0 = 2 */
+    uint32_t max_batch;     /* queries per batch; 0 = 256 */
+} kc_server_config;
+
+typedef struct {
+    uint64_t submitted, completed;  /* completed since the last kc_server_reset_stats */
+    uint64_t batches;
+    double mean_batch;              /* queries per batch */
+    double p50_ms, p99_ms, mean_ms; /* submit -> result latency */
+    double queries_per_s;           /* completed / (last completion - first submit) */
+    uint64_t graphs, resident_bytes, held_scratch_bytes;
+} kc_server_stats;
+
+/*
+ * Serving loop (no reference counterpart).  Graphs stay oriented in HBM;
+ * queries are queued by kc_server_submit and counted by worker threads that
+ * batch every pending query that fits the memory budget into persistent
+ * launches (continuous batching; batches of different workers overlap on the
+ * device).  A result is exactly what kc_count returns for the query alone.
+ */
+int kc_server_create(const kc_server_config *cfg, kc_server **out);
+/* Orient g and keep it resident; KC_ERR_BUDGET when it does not fit. */
+int kc_server_add_graph(kc_server *s, const kc_graph *g, int orient_mode, uint32_t *gid);
+/* Drop a graph (after its queued queries finish). */
+int kc_server_remove_graph(kc_server *s, uint32_t gid);
+/* Queue a query; returns at once with a ticket. */
+int kc_server_submit(kc_server *s, uint32_t gid, const kc_query *q, uint64_t *ticket);
+/* *done = 1 and *r filled when the ticket's result is ready (then the ticket is consumed). */
+int kc_server_poll(kc_server *s, uint64_t ticket, kc_result *r, int *done);
+/* Block until the ticket's result is ready. */
+int kc_server_wait(kc_server *s, uint64_t ticket, kc_result *r);
+/* Block until every submitted query has completed. */
+int kc_server_drain(kc_server *s);
+int kc_server_get_stats(kc_server *s, kc_server_stats *st);
+int kc_server_reset_stats(kc_server *s);
+void kc_server_destroy(kc_server *s);
+
 /* ---- device ------------------------------------------------------------ */
 typedef struct {
     char name[64];

@@ -82,6 +82,27 @@ class Result(ctypes.Structure):
         return d
 
 
+class Plan(ctypes.Structure):
+    _fields_ = [("orient_mode", ctypes.c_int), ("algo", ctypes.c_uint32), ("gran", ctypes.c_uint32),
+                ("part_size", ctypes.c_uint32), ("degree_max_out", ctypes.c_uint32)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_uint32), ("max_batch", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("graphs", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_uint64), ("held_scratch_bytes", ctypes.c_uint64)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("count_ms", ctypes.c_double), ("tasks", ctypes.c_uint64),
+                ("queries", ctypes.c_uint64)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32), ("cu_count", ctypes.c_int),
                 ("hbm_bytes", ctypes.c_uint64), ("lds_per_block", ctypes.c_uint64), ("clock_khz", ctypes.c_int)]
@@ -110,6 +131,19 @@ SIGNATURES = {
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
     "kc_select_device": (ctypes.c_int, [ctypes.c_int]),
+    "kc_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    "kc_server_add_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p]),
+    "kc_server_remove_graph": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "kc_server_submit": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_uint64)]),
+    "kc_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_int)]),
+    "kc_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(Result)]),
+    "kc_server_drain": (ctypes.c_int, [_vp]),
+    "kc_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "kc_server_reset_stats": (ctypes.c_int, [_vp]),
+    "kc_server_destroy": (None, [_vp]),
+    "kc_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(Plan)]),
+    "kc_batch_count": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(Query), ctypes.c_size_t,
+                                      ctypes.POINTER(Result), ctypes.POINTER(BatchStats)]),
 }
 
 _lib = None
@@ -196,6 +230,13 @@ class Graph:
         _check(lib().kc_graph_get_csr(self._h, rp.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p)))
         return rp, col[: inf["nnz"]]
 
+    def autotune(self, k):
+        """The library's plan: dict(orient, algo, gran, part_size, degree_max_out)."""
+        p = Plan()
+        _check(lib().kc_autotune(self._h, int(k), ctypes.byref(p)))
+        return {"orient": "degen" if p.orient_mode == 1 else "degree", "algo": "pivot" if p.algo else "orient",
+                "gran": "edge" if p.gran else "node", "part_size": p.part_size, "degree_max_out": p.degree_max_out}
+
     def orient(self, mode="degree"):
         h = ctypes.c_void_p()
         _check(lib().kc_orient(self._h, ORIENT[mode] if isinstance(mode, str) else int(mode), ctypes.byref(h)))
@@ -266,6 +307,86 @@ class Oriented:
         if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
             _check(rc)
         return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+def _query(k, algo="pivot", gran="node", part_size=1, binary=True, auto_tune=False):
+    return Query(int(k), ALGO[algo] if isinstance(algo, str) else int(algo),
+                 GRAN[gran] if isinstance(gran, str) else int(gran), int(part_size), int(bool(binary)),
+                 int(bool(auto_tune)))
+
+
+def batch_count(pairs, check=True):
+    """Count many (Oriented, query-dict) pairs in one persistent launch.
+
+    ``pairs`` is a list of ``(oriented, dict(k=..., algo=..., gran=..., part_size=..., auto_tune=...))``.
+    Returns ``(list of CountResult, stats dict)``."""
+    n = len(pairs)
+    hs = (_vp * max(n, 1))(*[o._h.value for o, _ in pairs])
+    qs = (Query * max(n, 1))(*[_query(**q) for _, q in pairs])
+    rs = (Result * max(n, 1))()
+    st = BatchStats()
+    rc = lib().kc_batch_count(hs, qs, n, rs, ctypes.byref(st))
+    if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
+        _check(rc)
+    out = [CountResult(int(rs[i].count), bool(rs[i].overflow), rs[i].as_dict()) for i in range(n)]
+    return out, {"total_ms": st.total_ms, "count_ms": st.count_ms, "tasks": st.tasks, "queries": st.queries}
+
+
+class Server:
+    """Serving loop: resident oriented graphs, queued (graph, k) queries,
+    continuous batching on the GPU (kc_server_* in include/kc.h)."""
+
+    def __init__(self, budget_bytes=0, workers=0, max_batch=0):
+        self._h = ctypes.c_void_p()
+        cfg = ServerConfig(int(budget_bytes), int(workers), int(max_batch))
+        _check(lib().kc_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.kc_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    __del__ = close
+
+    def add_graph(self, graph, orient="degen"):
+        gid = ctypes.c_uint32()
+        _check(lib().kc_server_add_graph(self._h, graph._h, ORIENT[orient], ctypes.byref(gid)))
+        return gid.value
+
+    def remove_graph(self, gid):
+        _check(lib().kc_server_remove_graph(self._h, int(gid)))
+
+    def submit(self, gid, k, algo="pivot", gran="node", part_size=1, auto_tune=False):
+        q = _query(k, algo, gran, part_size, True, auto_tune)
+        t = ctypes.c_uint64()
+        _check(lib().kc_server_submit(self._h, int(gid), ctypes.byref(q), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        r = Result()
+        done = ctypes.c_int()
+        rc = lib().kc_server_poll(self._h, int(ticket), ctypes.byref(r), ctypes.byref(done))
+        if rc not in (KC_OK, KC_ERR_OVERFLOW):
+            _check(rc)
+        return CountResult(int(r.count), bool(r.overflow), r.as_dict()) if done.value else None
+
+    def wait(self, ticket):
+        r = Result()
+        rc = lib().kc_server_wait(self._h, int(ticket), ctypes.byref(r))
+        if rc not in (KC_OK, KC_ERR_OVERFLOW):
+            _check(rc)
+        return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+    def drain(self):
+        _check(lib().kc_server_drain(self._h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().kc_server_get_stats(self._h, ctypes.byref(st)))
+        return {f[0]: getattr(st, f[0]) for f in st._fields_}
+
+    def reset_stats(self):
+        _check(lib().kc_server_reset_stats(self._h))
 
 
 def device_info():

@@ -23,7 +23,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 CPP = ["kc_io.cpp", "kc_gen.cpp", "kc_api.cpp"]
-HIP = ["kc_prep.hip", "kc_count.hip"]
+HIP = ["kc_prep.hip", "kc_count.hip", "kc_serve.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -124,6 +124,37 @@ int kc_graph_get_csr(const kc_graph *g, uint32_t *rowptr, uint32_t *col)
 
 void kc_graph_free(kc_graph *g) { delete g; }
 
+// Rules measured on gfx950 (kc_count on generated graphs): pivoting with
+// closed-form tails is never slower than the orientation DFS, one lane per
+// candidate (P = 1) is the fastest mapping, a task whose vertex set fits one
+// 64-bit word runs in the smallest register kernel.  So: keep the cheap degree
+// orientation when it already bounds every out-degree by 64 (no peel needed),
+// otherwise peel for the degeneracy order, which bounds out-degrees by the
+// max core; split work per edge when out-degrees are heavy-tailed.
+int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan)
+{
+    if (!g || !plan) return fail(KC_ERR_ARG, "kc_autotune: null argument");
+    (void)k;
+    const HostCSR &c = g->csr;
+    uint32_t maxout = 0;
+#pragma omp parallel for reduction(max : maxout) schedule(static, 4096)
+    for (uint32_t u = 0; u < c.n; ++u) {
+        const uint32_t du = c.rowptr[u + 1] - c.rowptr[u];
+        uint32_t out = 0;
+        for (uint32_t e = c.rowptr[u]; e < c.rowptr[u + 1]; ++e) {
+            const uint32_t v = c.col[e], dv = c.rowptr[v + 1] - c.rowptr[v];
+            out += (du < dv || (du == dv && u < v));
+        }
+        maxout = out > maxout ? out : maxout;
+    }
+    plan->degree_max_out = maxout;
+    plan->orient_mode = maxout <= 64 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
+    plan->algo = KC_ALGO_PIVOT;
+    plan->gran = maxout <= 64 ? KC_GRAN_NODE : KC_GRAN_EDGE;
+    plan->part_size = 1;
+    return KC_OK;
+}
+
 // -q grammar (include/Config.h:235-271): [o|p]<P>[b|n|h]
 int kc_parse_spec(const char *spec, kc_query *q)
 {

@@ -43,25 +43,35 @@ constexpr uint32_t CLASS_ROWS[NCLASS] = {64, 128, 256, 512, 1024, 0xFFFFFFFFu};
 constexpr uint32_t INF32 = 0xFFFFFFFFu;
 constexpr int ACC = 8;  // per-class accumulator slots
 
+// one query of a launch: the oriented graph it runs on and its k
+struct QDesc {
+    const uint32_t *rowptr, *col, *src;
+    uint32_t k, gran, pad0, pad1;
+};
+
+// the graph and k of the task a wave is working on
+struct Cur {
+    const uint32_t *rowptr, *col, *src;
+    uint32_t k, gran;
+};
+
 struct CountArgs {
-    const uint32_t *rowptr;
-    const uint32_t *col;
-    const uint32_t *src;
-    const uint32_t *tasks;
+    const QDesc *qd;
+    const uint64_t *tasks;  // (query << 32) | vertex-or-edge id
     uint32_t ntasks;
-    uint32_t k;
-    uint32_t psize;    // lanes per partition
-    uint32_t nparts;   // partitions per wave used by the orientation DFS
-    uint32_t chunk;    // tasks per queue grab
-    uint32_t rmax;     // row capacity of the workspace
-    uint32_t wmax;     // words per row capacity
-    uint32_t levels;   // stack levels
+    uint32_t psize;         // lanes per partition
+    uint32_t nparts;        // partitions per wave used by the orientation DFS
+    uint32_t chunk;         // tasks per queue grab
+    uint32_t rmax;          // row capacity of the workspace
+    uint32_t wmax;          // words per row capacity
+    uint32_t levels;        // stack levels
     const uint64_t *binom;
     const uint8_t *binom_ovf;
-    uint32_t bstride;  // binom[n * bstride + r]
+    uint32_t bstride;       // binom[n * bstride + r]
     uint32_t *qhead;
-    uint64_t *acc;     // [0] count [1] bytes [2] tasks [3] tree nodes [4] max task ticks [5] sum task ticks
-    uint32_t *ovf;
+    uint64_t *acc;          // [0] count [1] bytes [2] tasks [3] tree nodes [4] max task ticks [5] sum task ticks
+    uint64_t *qcount;       // per query
+    uint32_t *qovf;         // per query
     uint64_t *gscr;
     uint64_t gscr_words;
 };
@@ -185,8 +195,9 @@ __device__ inline uint32_t lower_bound_u32(const P *a, uint32_t n, uint32_t x)
 // ---- step 2: bitmap rows of the induced subgraph on list[0..R) -------------
 // bit j of row i <=> list[j] in N+(list[i]); FULL also sets bit i of row j.
 template <bool FULL>
-__device__ void build_rows(const CountArgs &a, uint32_t *list, uint32_t *offs, uint32_t *starts, uint64_t *rows,
-                           uint32_t R, uint32_t W, uint64_t &bytes)
+__device__ __attribute__((always_inline)) void build_rows(const Cur &g, uint32_t *list, uint32_t *offs,
+                                                         uint32_t *starts, uint64_t *rows, uint32_t R, uint32_t W,
+                                                         uint64_t &bytes)
 {
     const int lane = lane_id();
     for (uint32_t i = lane; i < R * W; i += 64) rows[i] = 0;
@@ -195,8 +206,8 @@ __device__ void build_rows(const CountArgs &a, uint32_t *list, uint32_t *offs, u
         uint32_t i = base + lane, d = 0;
         if (i < R) {
             uint32_t w = list[i];
-            uint32_t s = a.rowptr[w];
-            d = a.rowptr[w + 1] - s;
+            uint32_t s = g.rowptr[w];
+            d = g.rowptr[w + 1] - s;
             starts[i] = s;
         }
         uint32_t inc = wave_incl_scan32(d);
@@ -216,7 +227,7 @@ __device__ void build_rows(const CountArgs &a, uint32_t *list, uint32_t *offs, u
             if (offs[mid] <= f) lo = mid;
             else hi = mid - 1;
         }
-        uint32_t x = a.col[starts[lo] + (f - offs[lo])];
+        uint32_t x = g.col[starts[lo] + (f - offs[lo])];
         if (x < lo_v || x > hi_v) continue;
         uint32_t j = lower_bound_u32(list, R, x);
         if (j < R && list[j] == x) {
@@ -321,12 +332,13 @@ struct Scal {
 // closes h = k-3 with c_3 = triangles of G[X].  A node is also dropped when
 // h + p + max_j|X & row_j| + 1 < k (the largest clique inside X has at most
 // maxIntersection + 1 vertices, cf. kclique/kckernels.cuh:2573-2584).
-__device__ uint64_t pivot_count(const CountArgs &a, const uint64_t *rows, uint64_t *stk, uint64_t *X, Scal *scal,
-                                uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes, uint32_t &ovf)
+__device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
+                                Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
+                                uint32_t &ovf)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    const uint32_t k = a.k, wmax = a.wmax;
+    const uint32_t wmax = a.wmax;
     const uint32_t P = a.psize, NPp = 64 / P, part = lane / P, lip = lane % P;
     uint64_t acc = 0;
     // root candidate set = all R list vertices
@@ -470,12 +482,11 @@ __device__ inline uint64_t lds_row_word(const uint64_t *rows, uint32_t W, uint32
 }
 
 template <int S>
-__device__ uint64_t pivot_reg(const CountArgs &a, const uint64_t *rows, uint32_t R, uint32_t h0, uint64_t &nodes,
-                              uint32_t &ovf)
+__device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
+                              uint64_t &nodes, uint32_t &ovf)
 {
     constexpr int NB = (S == 1) ? 7 : (S == 2) ? 8 : 9;  // bits of (score + 1) <= 64*S
     const uint32_t lane = (uint32_t)lane_id();
-    const uint32_t k = a.k;
     const uint32_t W = (R + 63) / 64;
     uint64_t row[S][S];
 #pragma unroll
@@ -644,22 +655,22 @@ __device__ uint64_t pivot_reg(const CountArgs &a, const uint64_t *rows, uint32_t
 // ---- step 1: stage a task's vertex list in list[] ------------------------------
 // node task: N+(u); edge task: T = N+(u) & N+(v), walking the shorter list and
 // binary-searching the longer one, compacted with a ballot (ascending order).
-template <bool EDGE>
-__device__ inline uint32_t stage_task(const CountArgs &a, uint32_t task, uint32_t *list, uint64_t &bytes)
+__device__ __attribute__((always_inline)) uint32_t stage_task(const Cur &g, uint32_t task, uint32_t *list,
+                                                              uint64_t &bytes)
 {
     const int lane = lane_id();
-    if (!EDGE) {
-        const uint32_t b = a.rowptr[task], e = a.rowptr[task + 1];
+    if (!g.gran) {
+        const uint32_t b = g.rowptr[task], e = g.rowptr[task + 1];
         const uint32_t R = e - b;
         bytes += 8 + 4ull * R;
-        for (uint32_t i = lane; i < R; i += 64) list[i] = a.col[b + i];
+        for (uint32_t i = lane; i < R; i += 64) list[i] = g.col[b + i];
         return R;
     }
-    const uint32_t u = a.src[task], v = a.col[task];
-    const uint32_t ub = a.rowptr[u], ue = a.rowptr[u + 1], vb = a.rowptr[v], ve = a.rowptr[v + 1];
+    const uint32_t u = g.src[task], v = g.col[task];
+    const uint32_t ub = g.rowptr[u], ue = g.rowptr[u + 1], vb = g.rowptr[v], ve = g.rowptr[v + 1];
     const uint32_t du = ue - ub, dv = ve - vb;
     bytes += 24 + 4ull * (du + dv);
-    const uint32_t *sa = a.col + (du <= dv ? ub : vb), *sb = a.col + (du <= dv ? vb : ub);
+    const uint32_t *sa = g.col + (du <= dv ? ub : vb), *sb = g.col + (du <= dv ? vb : ub);
     const uint32_t la = min(du, dv), lb = max(du, dv);
     uint32_t cnt = 0;
     for (uint32_t i0 = 0; i0 < la; i0 += 64) {
@@ -678,8 +689,60 @@ __device__ inline uint32_t stage_task(const CountArgs &a, uint32_t task, uint32_
     return cnt;
 }
 
+// Per-wave bookkeeping shared by the kernels: the graph of the current task,
+// the running count of the current query (flushed to qcount[] when the query
+// changes and at the end), and the launch statistics.
+struct WaveAcc {
+    uint32_t q = INF32;
+    uint64_t run = 0;
+    uint32_t run_ovf = 0;
+    uint64_t bytes = 0, ntask = 0, nodes = 0, tmax = 0, tsum = 0;
+    uint32_t any_ovf = 0;
+};
+
+__device__ __attribute__((always_inline)) void flush_query(const CountArgs &a, WaveAcc &w)
+{
+    if (w.q == INF32) return;
+    if (lane_id() == 0) {
+        unsigned long long old = atomicAdd((unsigned long long *)&a.qcount[w.q], (unsigned long long)w.run);
+        if (old + w.run < old || w.run_ovf) {
+            atomicOr(&a.qovf[w.q], 1u);
+        }
+    }
+    w.any_ovf |= w.run_ovf;
+    w.run = 0;
+    w.run_ovf = 0;
+}
+
+__device__ __attribute__((always_inline)) Cur select_task(const CountArgs &a, uint64_t code, WaveAcc &w,
+                                                          uint32_t &task)
+{
+    const uint32_t q = (uint32_t)(code >> 32);
+    task = (uint32_t)code;
+    if (q != w.q) {
+        flush_query(a, w);
+        w.q = q;
+    }
+    const QDesc d = a.qd[q];
+    return Cur{d.rowptr, d.col, d.src, d.k, d.gran};
+}
+
+__device__ __attribute__((always_inline)) void finish_wave(const CountArgs &a, WaveAcc &w)
+{
+    flush_query(a, w);
+    uint32_t nd_wrap = 0;
+    const uint64_t nd = wave_sum64_checked(w.nodes, nd_wrap);  // tree nodes are per-lane shares
+    if (lane_id() == 0) {
+        atomicAdd((unsigned long long *)&a.acc[1], (unsigned long long)w.bytes);
+        atomicAdd((unsigned long long *)&a.acc[2], (unsigned long long)w.ntask);
+        atomicAdd((unsigned long long *)&a.acc[3], (unsigned long long)nd);
+        atomicMax((unsigned long long *)&a.acc[4], (unsigned long long)w.tmax);
+        atomicAdd((unsigned long long *)&a.acc[5], (unsigned long long)w.tsum);
+    }
+}
+
 // register-bitmap pivoting kernel for tasks of at most 64*S vertices
-template <bool EDGE, int S>
+template <int S>
 __global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
@@ -689,8 +752,7 @@ __global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
     uint32_t *offs = reinterpret_cast<uint32_t *>(lds + L.offs);
     uint32_t *starts = reinterpret_cast<uint32_t *>(lds + L.starts);
     uint64_t *rows = lds + L.rows;
-    uint64_t total = 0, bytes = 0, ntask = 0, nodes = 0, tmax = 0, tsum = 0;
-    uint32_t ovf = 0;
+    WaveAcc w;
     for (;;) {
         uint32_t start = 0;
         if (lane == 0) start = atomicAdd(a.qhead, a.chunk);
@@ -699,34 +761,24 @@ __global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
         const uint32_t stop = min(a.ntasks, start + a.chunk);
         for (uint32_t ti = start; ti < stop; ++ti) {
             const uint64_t t0 = wall_clock64();
-            ++ntask;
-            const uint32_t R = stage_task<EDGE>(a, a.tasks[ti], list, bytes);
+            uint32_t task;
+            const Cur g = select_task(a, a.tasks[ti], w, task);
+            ++w.ntask;
+            const uint32_t R = stage_task(g, task, list, w.bytes);
             __syncthreads();
-            build_rows<true>(a, list, offs, starts, rows, R, (R + 63) / 64, bytes);
-            add_checked(total, pivot_reg<S>(a, rows, R, EDGE ? 2u : 1u, nodes, ovf), ovf);
+            build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
+            add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, w.nodes, w.run_ovf), w.run_ovf);
             __syncthreads();
             const uint64_t dt = wall_clock64() - t0;
-            tmax = dt > tmax ? dt : tmax;
-            tsum += dt;
+            w.tmax = dt > w.tmax ? dt : w.tmax;
+            w.tsum += dt;
         }
     }
-    uint32_t novf = wave_min32(ovf ? 0u : 1u) == 0 ? 1u : 0u;
-    uint32_t nd_wrap = 0;
-    const uint64_t nd = wave_sum64_checked(nodes, nd_wrap);
-    if (lane == 0) {
-        unsigned long long old = atomicAdd((unsigned long long *)&a.acc[0], (unsigned long long)total);
-        if (old + total < old) novf = 1;
-        atomicAdd((unsigned long long *)&a.acc[1], (unsigned long long)bytes);
-        atomicAdd((unsigned long long *)&a.acc[2], (unsigned long long)ntask);
-        atomicAdd((unsigned long long *)&a.acc[3], (unsigned long long)nd);
-        atomicMax((unsigned long long *)&a.acc[4], (unsigned long long)tmax);
-        atomicAdd((unsigned long long *)&a.acc[5], (unsigned long long)tsum);
-        if (novf) atomicOr(a.ovf, 1u);
-    }
+    finish_wave(a, w);
 }
 
-// ---- the kernel ---------------------------------------------------------------
-template <bool PIVOT, bool EDGE, bool GLOBAL>
+// ---- the generic kernel: LDS (or HBM) bitmap, orientation or pivoting ------
+template <bool PIVOT, bool GLOBAL>
 __global__ __launch_bounds__(64) void k_count(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
@@ -741,10 +793,7 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
     uint64_t *stk = ws + L.stk;
     uint64_t *X = ws + L.x;
     Scal *scal = reinterpret_cast<Scal *>(ws + L.scal);
-
-    const uint32_t k = a.k;
-    uint64_t total = 0, bytes = 0, ntask = 0, nodes = 0;
-    uint32_t ovf = 0;
+    WaveAcc w;
     for (;;) {
         uint32_t start = 0;
         if (lane == 0) start = atomicAdd(a.qhead, a.chunk);
@@ -752,10 +801,14 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
         if (start >= a.ntasks) break;
         const uint32_t stop = min(a.ntasks, start + a.chunk);
         for (uint32_t ti = start; ti < stop; ++ti) {
-            ++ntask;
-            const uint32_t R = stage_task<EDGE>(a, a.tasks[ti], list, bytes);
-            if (EDGE && !PIVOT && k == 3) {  // edge orientation at k=3 adds |T|
-                add_checked(total, R, ovf);
+            const uint64_t t0 = wall_clock64();
+            uint32_t task;
+            const Cur g = select_task(a, a.tasks[ti], w, task);
+            const uint32_t k = g.k;
+            ++w.ntask;
+            const uint32_t R = stage_task(g, task, list, w.bytes);
+            if (g.gran && !PIVOT && k == 3) {  // edge orientation at k=3 adds |T|
+                add_checked(w.run, R, w.run_ovf);
                 continue;
             }
             if (lane == 0) misc[0] = 0;
@@ -763,115 +816,131 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
             const uint32_t W = (R + 63) / 64;
             if (!PIVOT) {
                 // (k-1)-cliques of the DAG on N+(u) / (k-2)-cliques of the DAG on T
-                build_rows<false>(a, list, offs, starts, rows, R, W, bytes);
-                uint64_t part = orient_count(a, rows, stk, misc, R, W, EDGE ? k - 2 : k - 1, nodes, ovf);
-                uint64_t s = wave_sum64_checked(part, ovf);
-                add_checked(total, s, ovf);
+                build_rows<false>(g, list, offs, starts, rows, R, W, w.bytes);
+                uint32_t povf = 0;
+                uint64_t part = orient_count(a, rows, stk, misc, R, W, g.gran ? k - 2 : k - 1, w.nodes, povf);
+                uint64_t sum = wave_sum64_checked(part, povf);
+                w.run_ovf |= (wave_min32(povf ? 0u : 1u) == 0) ? 1u : 0u;
+                add_checked(w.run, sum, w.run_ovf);
             } else {
                 // SCT rooted at the node (held 1) or the edge (held 2)
-                build_rows<true>(a, list, offs, starts, rows, R, W, bytes);
-                uint64_t s = pivot_count(a, rows, stk, X, scal, offs, R, W, EDGE ? 2u : 1u, nodes, ovf);
-                add_checked(total, s, ovf);
+                build_rows<true>(g, list, offs, starts, rows, R, W, w.bytes);
+                uint64_t sum = pivot_count(a, k, rows, stk, X, scal, offs, R, W, g.gran ? 2u : 1u, w.nodes, w.run_ovf);
+                add_checked(w.run, sum, w.run_ovf);
             }
             __syncthreads();
+            const uint64_t dt = wall_clock64() - t0;
+            w.tmax = dt > w.tmax ? dt : w.tmax;
+            w.tsum += dt;
         }
     }
-    // all lanes hold the same `total`; one atomic per wave
-    uint32_t novf = wave_min32(ovf ? 0u : 1u) == 0 ? 1u : 0u;
-    uint32_t nd_wrap = 0;
-    const uint64_t nd = wave_sum64_checked(nodes, nd_wrap);  // tree nodes are per-lane shares
-    if (lane == 0) {
-        unsigned long long old = atomicAdd((unsigned long long *)&a.acc[0], (unsigned long long)total);
-        if (old + total < old) novf = 1;
-        atomicAdd((unsigned long long *)&a.acc[1], (unsigned long long)bytes);
-        atomicAdd((unsigned long long *)&a.acc[2], (unsigned long long)ntask);
-        atomicAdd((unsigned long long *)&a.acc[3], (unsigned long long)nd);
-        if (novf) atomicOr(a.ovf, 1u);
-    }
+    finish_wave(a, w);
 }
 
-// |T| = |N+(u) & N+(v)| for every oriented edge (one lane per edge, merge)
-__global__ __launch_bounds__(256) void k_tsize(const uint32_t *rowptr, const uint32_t *col, const uint32_t *src,
-                                                uint32_t m, uint32_t *tsz)
+// ---- task sizes and classes over the concatenated task space of a launch ----
+// Query q owns global task ids [qoff[q], qoff[q+1]): its vertices (node
+// granularity) or its oriented edges (edge granularity).
+__device__ inline uint32_t query_of(const uint64_t *qoff, uint32_t nq, uint64_t g)
 {
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = src[e], v = col[e];
-        uint32_t i = rowptr[u], ie = rowptr[u + 1], j = rowptr[v], je = rowptr[v + 1];
+    uint32_t lo = 0, hi = nq - 1;  // last q with qoff[q] <= g
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (qoff[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// size of a task: d+(u) for a node, |T| = |N+(u) & N+(v)| for an edge (merge)
+__global__ __launch_bounds__(256) void k_task_size(const QDesc *qd, const uint64_t *qoff, uint32_t nq, uint64_t total,
+                                                    uint32_t *size)
+{
+    for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
+         gi += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = query_of(qoff, nq, gi);
+        const QDesc d = qd[q];
+        const uint32_t t = (uint32_t)(gi - qoff[q]);
+        if (!d.gran) {
+            size[gi] = d.rowptr[t + 1] - d.rowptr[t];
+            continue;
+        }
+        const uint32_t u = d.src[t], v = d.col[t];
+        uint32_t i = d.rowptr[u], ie = d.rowptr[u + 1], j = d.rowptr[v], je = d.rowptr[v + 1];
         uint32_t c = 0;
         while (i < ie && j < je) {
-            const uint32_t x = col[i], y = col[j];
+            const uint32_t x = d.col[i], y = d.col[j];
             c += (x == y);
             i += (x <= y);
             j += (y <= x);
         }
-        tsz[e] = c;
+        size[gi] = c;
     }
 }
 
-// ---- task classification ----------------------------------------------------
 __device__ inline int class_of(uint32_t s)
 {
     return s <= 64 ? 0 : s <= 128 ? 1 : s <= 256 ? 2 : s <= 512 ? 3 : s <= 1024 ? 4 : 5;
 }
 
-// size of a task: d+(u) for a node, |T| (from k_tsize) for an edge; tasks
-// below `need` cannot hold a k-clique and are dropped
-template <bool EDGE>
-__global__ __launch_bounds__(256) void k_classify(const uint32_t *rowptr, const uint32_t *tsz,
-                                                   uint32_t ntask, uint32_t need, uint32_t *cls_cnt, uint32_t *cls_max,
-                                                   uint32_t *cls_pos, uint32_t *out, const uint32_t *cls_off,
-                                                   int scatter)
+// Bin tasks by (algorithm, size class); a task smaller than the clique part
+// still to find (k-1 for a node, k-2 for an edge) is dropped.  Pass 0 counts
+// and takes the max size per bin, pass 1 scatters (query << 32 | task).
+__global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t *qalgo, const uint64_t *qoff,
+                                                   uint32_t nq, uint64_t total, const uint32_t *size, uint32_t *bin_cnt,
+                                                   uint32_t *bin_max, uint32_t *bin_pos, const uint32_t *bin_off,
+                                                   uint64_t *out, int scatter)
 {
     const int lane = threadIdx.x & 63;
-    for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; base < ntask;
+    for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; base < total;
          base += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = base + lane;
+        const uint64_t gi = base + lane;
         int c = -1;
-        uint32_t s = 0;
-        if (t < ntask) {
-            if (EDGE) {
-                s = tsz[t];
-            } else {
-                s = rowptr[t + 1] - rowptr[t];
-            }
-            if (s >= need) c = class_of(s);
+        uint32_t s = 0, q = 0;
+        if (gi < total) {
+            q = query_of(qoff, nq, gi);
+            const QDesc d = qd[q];
+            const uint32_t need = d.gran ? d.k - 2 : d.k - 1;
+            s = size[gi];
+            if (s >= need) c = class_of(s) + (qalgo[q] ? NCLASS : 0);
         }
-        for (int q = 0; q < NCLASS; ++q) {
-            uint64_t bal = __ballot(c == q);
+        for (int b = 0; b < 2 * NCLASS; ++b) {
+            const uint64_t bal = __ballot(c == b);
             if (!bal) continue;
-            int leader = __ffsll((unsigned long long)bal) - 1;
+            const int leader = __ffsll((unsigned long long)bal) - 1;
             if (!scatter) {
-                uint32_t mx = (c == q) ? s : 0;
-                for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+                uint32_t mx = (c == b) ? s : 0;
+                for (int dd = 32; dd >= 1; dd >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, dd, 64));
                 if (lane == leader) {
-                    atomicAdd(&cls_cnt[q], (uint32_t)__popcll(bal));
-                    atomicMax(&cls_max[q], mx);
+                    atomicAdd(&bin_cnt[b], (uint32_t)__popcll(bal));
+                    atomicMax(&bin_max[b], mx);
                 }
             } else {
                 uint32_t b0 = 0;
-                if (lane == leader) b0 = atomicAdd(&cls_pos[q], (uint32_t)__popcll(bal));
+                if (lane == leader) b0 = atomicAdd(&bin_pos[b], (uint32_t)__popcll(bal));
                 b0 = __shfl(b0, leader, 64);
-                if (c == q) out[cls_off[q] + b0 + (uint32_t)__popcll(bal & lanemask_lt())] = (uint32_t)t;
+                if (c == b)
+                    out[bin_off[b] + b0 + (uint32_t)__popcll(bal & lanemask_lt())] =
+                        ((uint64_t)q << 32) | (uint32_t)(gi - qoff[q]);
             }
         }
     }
 }
 
-// ---- host ----------------------------------------------------------------------
-template <bool PIVOT, bool EDGE, bool GLOBAL>
-static const void *kernel_ptr()
+// ---- host -------------------------------------------------------------------
+static const void *generic_kernel(bool pivot, bool global)
 {
-    return reinterpret_cast<const void *>(&k_count<PIVOT, EDGE, GLOBAL>);
+    if (pivot) return global ? reinterpret_cast<const void *>(&k_count<true, true>)
+                             : reinterpret_cast<const void *>(&k_count<true, false>);
+    return global ? reinterpret_cast<const void *>(&k_count<false, true>)
+                  : reinterpret_cast<const void *>(&k_count<false, false>);
 }
 
-static const void *pick_kernel(bool pivot, bool edge, bool global)
+static const void *register_kernel(int c)
 {
-    if (pivot) {
-        if (edge) return global ? kernel_ptr<true, true, true>() : kernel_ptr<true, true, false>();
-        return global ? kernel_ptr<true, false, true>() : kernel_ptr<true, false, false>();
-    }
-    if (edge) return global ? kernel_ptr<false, true, true>() : kernel_ptr<false, true, false>();
-    return global ? kernel_ptr<false, false, true>() : kernel_ptr<false, false, false>();
+    static const void *const reg[3] = {reinterpret_cast<const void *>(&k_pivot_reg<1>),
+                                       reinterpret_cast<const void *>(&k_pivot_reg<2>),
+                                       reinterpret_cast<const void *>(&k_pivot_reg<4>)};
+    return reg[c];
 }
 
 // exact C(n, r) for n <= nmax, r <= rmax; ovf[i] = 1 when it exceeds 2^64-1
@@ -891,252 +960,364 @@ static void binom_table(uint32_t nmax, uint32_t rmax, std::vector<uint64_t> &val
     }
 }
 
-struct ClassPlan {
-    uint32_t count = 0, maxsize = 0, offset = 0;
-};
-
-int count_impl(const kc_oriented *o, const kc_query *q, kc_result *r)
+int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
+                LaunchStats *ls)
 {
     DeviceCtx *ctx;
     int rc = device_ctx(&ctx);
     if (rc) return rc;
-    kc_oriented *om = const_cast<kc_oriented *>(o);
-    hipStream_t s = om->stream;
-    const uint32_t k = q->k;
-    const bool pivot = q->algo == KC_ALGO_PIVOT, edge = q->gran == KC_GRAN_EDGE;
-    const uint32_t P = q->part_size;
-    const uint32_t ntask = edge ? (uint32_t)o->m : o->n;
-    const uint32_t need = edge ? k - 2 : k - 1;  // k >= 3 here
+    const uint32_t nq = (uint32_t)qs.size();
+    res.assign(nq, kc_result{});
+    if (nq == 0) return KC_OK;
+    constexpr int NB = 2 * NCLASS;  // (algo, size class) bins
 
-    // --- task classes (two passes: histogram + max, then scatter) ---
-    // meta: cnt[6] max[6] pos[6] off[6] qhead[6] ovf[1] pad, acc[6*4] (u64)
-    const size_t meta_u32 = 64;
-    const size_t meta_bytes = meta_u32 * 4 + sizeof(uint64_t) * NCLASS * ACC;
-    if ((rc = om->s_meta.ensure(meta_bytes))) return rc;
-    uint32_t *m32 = om->s_meta.as<uint32_t>();
-    uint32_t *cls_cnt = m32, *cls_max = m32 + 8, *cls_pos = m32 + 16, *cls_off = m32 + 24, *qhead = m32 + 32,
-             *dovf = m32 + 40;
-    uint64_t *dacc = reinterpret_cast<uint64_t *>(m32 + meta_u32);
-    KC_HIP(hipMemsetAsync(om->s_meta.p, 0, meta_bytes, s));
-    int cg = (int)std::min<uint64_t>(((uint64_t)ntask + 255) / 256, (uint64_t)ctx->cu * 8);
-    if (cg < 1) cg = 1;
-    uint32_t *tsz = nullptr;
-    if (edge && ntask > 0) {
-        if ((rc = om->s_tsz.ensure(sizeof(uint32_t) * (size_t)ntask))) return rc;
-        tsz = om->s_tsz.as<uint32_t>();
-        int tg = (int)std::min<uint64_t>(((uint64_t)ntask + 255) / 256, (uint64_t)ctx->cu * 16);
-        hipLaunchKernelGGL(k_tsize, dim3(tg), dim3(256), 0, s, o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(),
-                           o->src.as<uint32_t>(), ntask, tsz);
+    // --- query descriptors and the concatenated task space ---
+    std::vector<QDesc> hq(nq);
+    std::vector<uint8_t> halgo(nq);
+    std::vector<uint64_t> hoff(nq + 1, 0);
+    uint32_t kmax = 0, P = 1;
+    for (uint32_t i = 0; i < nq; ++i) {
+        const kc_oriented *o = qs[i].o;
+        const kc_query &q = qs[i].q;
+        hq[i] = QDesc{o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(), o->src.as<uint32_t>(), q.k, q.gran, 0, 0};
+        halgo[i] = (uint8_t)(q.algo == KC_ALGO_PIVOT);
+        hoff[i + 1] = hoff[i] + (q.gran == KC_GRAN_EDGE ? o->m : o->n);
+        kmax = std::max(kmax, q.k);
+        if (i == 0) P = q.part_size;
     }
-    auto classify = [&](int scatter) {
-        if (edge)
-            hipLaunchKernelGGL(k_classify<true>, dim3(cg), dim3(256), 0, s, o->rowptr.as<uint32_t>(), tsz, ntask,
-                               need, cls_cnt, cls_max, cls_pos, om->s_tasks.as<uint32_t>(), cls_off, scatter);
-        else
-            hipLaunchKernelGGL(k_classify<false>, dim3(cg), dim3(256), 0, s, o->rowptr.as<uint32_t>(), tsz, ntask,
-                               need, cls_cnt, cls_max, cls_pos, om->s_tasks.as<uint32_t>(), cls_off, scatter);
-    };
-    if (ntask > 0) classify(0);
+    const uint64_t total = hoff[nq];
+    if (total >= (1ull << 32) * 64) return fail(KC_ERR_LIMIT, "too many tasks in one launch");
+    const size_t qd_bytes = sizeof(QDesc) * nq, off_bytes = sizeof(uint64_t) * (nq + 1);
+    const size_t meta_u32 = 128;  // bin_cnt[16] bin_max[16] bin_pos[16] bin_off[16] qhead[16]
+    const size_t acc_bytes = sizeof(uint64_t) * NB * ACC;
+    const size_t q_bytes = sizeof(uint64_t) * nq + sizeof(uint32_t) * nq;
+    if ((rc = scr.qdesc.ensure(qd_bytes + off_bytes + nq + 16)) ||
+        (rc = scr.meta.ensure(meta_u32 * 4 + acc_bytes + q_bytes + 16)) ||
+        (rc = scr.size.ensure(sizeof(uint32_t) * (total + 1))))
+        return rc;
+    QDesc *dq = scr.qdesc.as<QDesc>();
+    uint64_t *doff = reinterpret_cast<uint64_t *>(scr.qdesc.as<uint8_t>() + qd_bytes);
+    uint8_t *dalgo = scr.qdesc.as<uint8_t>() + qd_bytes + off_bytes;
+    uint32_t *m32 = scr.meta.as<uint32_t>();
+    uint32_t *bin_cnt = m32, *bin_max = m32 + 16, *bin_pos = m32 + 32, *bin_off = m32 + 48, *qhead = m32 + 64;
+    uint64_t *dacc = reinterpret_cast<uint64_t *>(m32 + meta_u32);
+    uint64_t *qcount = dacc + NB * ACC;
+    uint32_t *qovf = reinterpret_cast<uint32_t *>(qcount + nq);
+    KC_HIP(hipMemcpyAsync(dq, hq.data(), qd_bytes, hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemcpyAsync(doff, hoff.data(), off_bytes, hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemcpyAsync(dalgo, halgo.data(), nq, hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemsetAsync(scr.meta.p, 0, meta_u32 * 4 + acc_bytes + q_bytes, s));
+
+    hipEvent_t e_start, e_cls;
+    hipEventCreate(&e_start);
+    hipEventCreate(&e_cls);
+    hipEventRecord(e_start, s);
+    const int gs = (int)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, (uint64_t)ctx->cu * 16));
+    if (total > 0) {
+        hipLaunchKernelGGL(k_task_size, dim3(gs), dim3(256), 0, s, dq, doff, nq, total, scr.size.as<uint32_t>());
+        hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
+                           bin_cnt, bin_max, bin_pos, bin_off, scr.tasks.as<uint64_t>(), 0);
+    }
     KC_HIP(hipGetLastError());
-    uint32_t h[16];
+    uint32_t h[32];
     KC_HIP(hipMemcpyAsync(h, m32, sizeof h, hipMemcpyDeviceToHost, s));
     KC_HIP(hipStreamSynchronize(s));
-    ClassPlan plan[NCLASS];
-    uint32_t tot = 0, maxsize = 0;
-    for (int c = 0; c < NCLASS; ++c) {
-        plan[c].count = h[c];
-        plan[c].maxsize = h[8 + c];
-        plan[c].offset = tot;
-        tot += h[c];
-        maxsize = std::max(maxsize, h[8 + c]);
+    uint32_t bcnt[NB], bmax[NB], boff[16] = {0};
+    uint64_t tot = 0;
+    uint32_t maxsize = 0;
+    for (int b = 0; b < NB; ++b) {
+        bcnt[b] = h[b];
+        bmax[b] = h[16 + b];
+        boff[b] = (uint32_t)tot;
+        tot += h[b];
+        maxsize = std::max(maxsize, h[16 + b]);
     }
-    if ((rc = om->s_tasks.ensure(sizeof(uint32_t) * ((size_t)tot + 1)))) return rc;
-    uint32_t hoff[NCLASS];
-    for (int c = 0; c < NCLASS; ++c) hoff[c] = plan[c].offset;
-    KC_HIP(hipMemcpyAsync(cls_off, hoff, sizeof hoff, hipMemcpyHostToDevice, s));
-    if (tot > 0) classify(1);
+    if (tot >= 0xFFFFFFFFull) return fail(KC_ERR_LIMIT, "more than 2^32 tasks in one launch");
+    if ((rc = scr.tasks.ensure(sizeof(uint64_t) * ((size_t)tot + 1)))) return rc;
+    KC_HIP(hipMemcpyAsync(bin_off, boff, sizeof boff, hipMemcpyHostToDevice, s));
+    if (tot > 0)
+        hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
+                           bin_cnt, bin_max, bin_pos, bin_off, scr.tasks.as<uint64_t>(), 1);
     KC_HIP(hipGetLastError());
+    hipEventRecord(e_cls, s);
 
-    // --- binomial table (pivoting) ---
-    const uint32_t bstride = k + 1;
-    if (pivot) {
+    // --- binomial table for the pivot leaves ---
+    const uint32_t bstride = kmax + 1;
+    const size_t nbin = (size_t)(maxsize + 2) * bstride;
+    {
         std::vector<uint64_t> bv;
         std::vector<uint8_t> bo;
-        binom_table(maxsize + 1, k, bv, bo);
-        size_t nb = bv.size();
-        if ((rc = om->s_binom.ensure(nb * 9 + 16))) return rc;
-        KC_HIP(hipMemcpyAsync(om->s_binom.p, bv.data(), nb * 8, hipMemcpyHostToDevice, s));
-        KC_HIP(hipMemcpyAsync(om->s_binom.as<uint8_t>() + nb * 8, bo.data(), nb, hipMemcpyHostToDevice, s));
-        KC_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+        binom_table(maxsize + 1, kmax, bv, bo);
+        if ((rc = scr.binom.ensure(nbin * 9 + 16))) return rc;
+        KC_HIP(hipMemcpyAsync(scr.binom.p, bv.data(), nbin * 8, hipMemcpyHostToDevice, s));
+        KC_HIP(hipMemcpyAsync(scr.binom.as<uint8_t>() + nbin * 8, bo.data(), nbin, hipMemcpyHostToDevice, s));
+        KC_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
     }
-    const size_t nbin = (size_t)(maxsize + 2) * bstride;
 
-    // --- launches, largest class first ---
-    hipEvent_t ev[NCLASS + 1];
+    // --- one persistent launch per non-empty bin, largest tasks first ---
+    hipEvent_t ev[NB];
     for (auto &e : ev) hipEventCreate(&e);
-    float cls_ms[NCLASS] = {0};
-    bool launched[NCLASS] = {false};
-    uint32_t variant[NCLASS] = {0};
-    hipEventRecord(ev[NCLASS], s);
+    uint32_t variant[NB] = {0};
+    bool launched[NB] = {false};
     const size_t lds_cap = ctx->lds_per_block;
+    int order[NB], no = 0;
     for (int c = NCLASS - 1; c >= 0; --c) {
-        if (plan[c].count == 0) continue;
+        order[no++] = NCLASS + c;  // pivot bins
+        order[no++] = c;           // orientation bins
+    }
+    hipEvent_t prev = e_cls;
+    for (int oi = 0; oi < NB; ++oi) {
+        const int b = order[oi];
+        if (bcnt[b] == 0) continue;
+        const bool pivot = b >= NCLASS;
+        const int c = b % NCLASS;
         CountArgs a{};
-        a.rowptr = o->rowptr.as<uint32_t>();
-        a.col = o->col.as<uint32_t>();
-        a.src = o->src.as<uint32_t>();
-        a.tasks = om->s_tasks.as<uint32_t>() + plan[c].offset;
-        a.ntasks = plan[c].count;
-        a.k = k;
+        a.qd = dq;
+        a.tasks = scr.tasks.as<uint64_t>() + boff[b];
+        a.ntasks = bcnt[b];
         a.psize = P;
-        a.binom = pivot ? om->s_binom.as<uint64_t>() : nullptr;
-        a.binom_ovf = pivot ? om->s_binom.as<uint8_t>() + nbin * 8 : nullptr;
+        a.binom = scr.binom.as<uint64_t>();
+        a.binom_ovf = scr.binom.as<uint8_t>() + nbin * 8;
         a.bstride = bstride;
-        a.qhead = qhead + c;
-        a.acc = dacc + ACC * c;
-        a.ovf = dovf;
+        a.qhead = qhead + b;
+        a.acc = dacc + ACC * b;
+        a.qcount = qcount;
+        a.qovf = qovf;
         bool global = (c == NCLASS - 1);
-        uint32_t rows_cap = global ? plan[c].maxsize : CLASS_ROWS[c];
+        const uint32_t rows_cap = global ? bmax[b] : CLASS_ROWS[c];
         a.rmax = rows_cap;
         a.wmax = (rows_cap + 63) / 64;
-        a.levels = pivot ? k + 3 : std::max<uint32_t>(1, k - 2);
+        a.levels = pivot ? kmax + 3 : std::max<uint32_t>(1, kmax - 2);
         uint32_t np = 64 / P;
         Layout L = make_layout(a.rmax, a.wmax, a.levels, np, pivot);
         if (!global && !pivot) {
-            // fewer partitions when the stacks would not fit in LDS
-            while (np > 1 && L.total * 8 > lds_cap) {
+            while (np > 1 && L.total * 8 > lds_cap) {  // fewer partitions when the stacks do not fit
                 np >>= 1;
                 L = make_layout(a.rmax, a.wmax, a.levels, np, pivot);
             }
         }
-        if (!global && L.total * 8 > lds_cap) {
-            global = true;  // does not fit: same layout in HBM
-        }
+        if (!global && L.total * 8 > lds_cap) global = true;  // same layout in HBM
         a.nparts = np;
-        const void *kern = pick_kernel(pivot, edge, global);
+        const void *kern = generic_kernel(pivot, global);
+        variant[b] = global ? 3u : 1u;
         if (pivot && !global && c <= 2) {
-            // register-bitmap SCT: rows live in VGPRs, LDS only for the build
-            static const void *const reg[2][3] = {
-                {reinterpret_cast<const void *>(&k_pivot_reg<false, 1>), reinterpret_cast<const void *>(&k_pivot_reg<false, 2>),
-                 reinterpret_cast<const void *>(&k_pivot_reg<false, 4>)},
-                {reinterpret_cast<const void *>(&k_pivot_reg<true, 1>), reinterpret_cast<const void *>(&k_pivot_reg<true, 2>),
-                 reinterpret_cast<const void *>(&k_pivot_reg<true, 4>)}};
-            kern = reg[edge ? 1 : 0][c];
+            kern = register_kernel(c);
             L = make_layout(a.rmax, a.wmax, 0, 0, true);
+            variant[b] = 2u;
         }
-        size_t lds = global ? 0 : L.total * 8;
+        const size_t lds = global ? 0 : L.total * 8;
         if (!global) KC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int per_cu = 0;
         KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
         if (per_cu < 1) per_cu = 1;
         uint64_t blocks = (uint64_t)per_cu * ctx->cu;
-        a.chunk = (c == 0 && plan[c].count > blocks * 64) ? 4 : 1;
-        blocks = std::min<uint64_t>(blocks, ((uint64_t)plan[c].count + a.chunk - 1) / a.chunk);
+        a.chunk = (c == 0 && bcnt[b] > blocks * 64) ? 4 : 1;
+        blocks = std::min<uint64_t>(blocks, ((uint64_t)bcnt[b] + a.chunk - 1) / a.chunk);
         if (global) {
             a.gscr_words = L.total;
-            uint64_t cap = (2ull << 30) / (L.total * 8 + 1);
+            const uint64_t cap = (2ull << 30) / (L.total * 8 + 1);
             blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
-            if ((rc = om->s_gscr.ensure(blocks * L.total * 8))) {
+            if ((rc = scr.gscr.ensure(blocks * L.total * 8))) {
                 for (auto &e : ev) hipEventDestroy(e);
                 return rc;
             }
-            a.gscr = om->s_gscr.as<uint64_t>();
+            a.gscr = scr.gscr.as<uint64_t>();
         }
         void *args[] = {&a};
         KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64), args, lds, s));
-        variant[c] = global ? 3u : (pivot && c <= 2) ? 2u : 1u;
-        hipEventRecord(ev[c], s);
-        launched[c] = true;
+        hipEventRecord(ev[b], s);
+        launched[b] = true;
     }
-    uint64_t hacc[NCLASS * ACC];
-    uint32_t hovf = 0;
+    std::vector<uint64_t> hq_count(nq);
+    std::vector<uint32_t> hq_ovf(nq);
+    uint64_t hacc[NB * ACC];
     KC_HIP(hipMemcpyAsync(hacc, dacc, sizeof hacc, hipMemcpyDeviceToHost, s));
-    KC_HIP(hipMemcpyAsync(&hovf, dovf, sizeof hovf, hipMemcpyDeviceToHost, s));
-    hipError_t se = hipStreamSynchronize(s);
+    KC_HIP(hipMemcpyAsync(hq_count.data(), qcount, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, s));
+    KC_HIP(hipMemcpyAsync(hq_ovf.data(), qovf, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, s));
+    const hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) {
         for (auto &e : ev) hipEventDestroy(e);
+        hipEventDestroy(e_start);
+        hipEventDestroy(e_cls);
         return fail(KC_ERR_DEVICE, "counting kernels failed: %s", hipGetErrorString(se));
     }
-    // per-class device times: each class starts where the previous launch ended
-    hipEvent_t prev = ev[NCLASS];
-    float total_ms = 0;
-    for (int c = NCLASS - 1; c >= 0; --c) {
-        if (!launched[c]) continue;
-        hipEventElapsedTime(&cls_ms[c], prev, ev[c]);
-        total_ms += cls_ms[c];
-        prev = ev[c];
+    float bin_ms[NB] = {0}, total_ms = 0, prep_ms = 0;
+    hipEventElapsedTime(&prep_ms, e_start, e_cls);
+    for (int oi = 0; oi < NB; ++oi) {
+        const int b = order[oi];
+        if (!launched[b]) continue;
+        hipEventElapsedTime(&bin_ms[b], prev, ev[b]);
+        total_ms += bin_ms[b];
+        prev = ev[b];
     }
     for (auto &e : ev) hipEventDestroy(e);
-
-    std::memset(r, 0, sizeof *r);
+    hipEventDestroy(e_start);
+    hipEventDestroy(e_cls);
     int wall_khz = 100000;
     hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ctx->dev);
     if (wall_khz <= 0) wall_khz = 100000;
-    uint64_t cnt = 0;
-    bool ov = hovf != 0;
-    int dom = -1;
-    for (int c = 0; c < NCLASS; ++c) {
-        uint64_t v = hacc[ACC * c];
-        if (cnt + v < cnt) ov = true;
-        cnt += v;
-        r->alg_bytes += hacc[ACC * c + 1];
-        r->tasks += hacc[ACC * c + 2];
-        r->tree_nodes += hacc[ACC * c + 3];
-        if (launched[c] && (dom < 0 || cls_ms[c] > cls_ms[dom])) dom = c;
-        r->class_ms[c] = cls_ms[c];
-        r->class_bytes[c] = hacc[ACC * c + 1];
-        r->class_tasks[c] = hacc[ACC * c + 2];
-        r->class_kernel[c] = variant[c];
-        r->class_max_task_ms[c] = (double)hacc[ACC * c + 4] / (double)wall_khz;
-        r->class_task_ms_sum[c] = (double)hacc[ACC * c + 5] / (double)wall_khz;
+
+    for (uint32_t i = 0; i < nq; ++i) {
+        kc_result &r = res[i];
+        r.count = hq_count[i];
+        r.overflow = hq_ovf[i] ? 1 : 0;
+        r.algo = qs[i].q.algo;
+        r.gran = qs[i].q.gran;
+        r.part_size = qs[i].q.part_size;
     }
-    r->count = cnt;
-    r->overflow = ov ? 1 : 0;
-    r->algo = q->algo;
-    r->gran = q->gran;
-    r->part_size = P;
-    r->count_ms = total_ms;
-    if (dom >= 0) {
-        r->dominant_ms = cls_ms[dom];
-        r->dominant_bytes = hacc[ACC * dom + 1];
-        r->dominant_class = (uint32_t)dom;
+    if (ls) {
+        *ls = LaunchStats{};
+        ls->count_ms = total_ms;
+        ls->prep_ms = prep_ms;
+        int dom = -1;
+        for (int b = 0; b < NB; ++b) {
+            ls->tasks += hacc[ACC * b + 2];
+            ls->alg_bytes += hacc[ACC * b + 1];
+            ls->tree_nodes += hacc[ACC * b + 3];
+            if (launched[b] && (dom < 0 || bin_ms[b] > bin_ms[dom])) dom = b;
+            const int c = b % NCLASS;
+            if (!launched[b]) continue;
+            ls->class_ms[c] += bin_ms[b];
+            ls->class_bytes[c] += hacc[ACC * b + 1];
+            ls->class_tasks[c] += hacc[ACC * b + 2];
+            ls->class_kernel[c] = variant[b];
+            ls->class_max_task_ms[c] = std::max(ls->class_max_task_ms[c], (double)hacc[ACC * b + 4] / wall_khz);
+            ls->class_task_ms_sum[c] += (double)hacc[ACC * b + 5] / wall_khz;
+        }
+        if (dom >= 0) {
+            ls->dominant_ms = bin_ms[dom];
+            ls->dominant_bytes = hacc[ACC * dom + 1];
+            ls->dominant_class = (uint32_t)(dom % NCLASS);
+        }
     }
-    return ov ? fail(KC_ERR_OVERFLOW, "k-clique count exceeds 2^64-1") : KC_OK;
+    return KC_OK;
 }
 
 }  // namespace kc
 
 using namespace kc;
 
+namespace kc {
+int check_query(const kc_query *q);
+}
+int kc::check_query(const kc_query *q)
+{
+    if (q->k < 1 || q->k > KC_MAX_K) return fail(KC_ERR_ARG, "k=%u outside 1..%d", q->k, KC_MAX_K);
+    if (q->algo > 1 || q->gran > 1) return fail(KC_ERR_ARG, "bad algo/gran");
+    const uint32_t P = q->part_size;
+    if (P == 0 || P > 64 || (P & (P - 1))) return fail(KC_ERR_ARG, "partition size %u", P);
+    return KC_OK;
+}
+
+// The autotuner (auto_tune = 1).  Measured on gfx950: one lane per candidate
+// row / per DFS partition (P = 1) is fastest for every size class; pivoting
+// with closed-form tails is never slower than the orientation DFS; edge tasks
+// balance heavy-tailed degree distributions, node tasks win when every
+// out-degree fits one 64-bit word and there are fewer vertices than edges to
+// schedule.
+namespace kc {
+kc_query tune_query(const kc_oriented *o, const kc_query *q);
+}
+static kc_query tune(const kc_oriented *o, const kc_query *q) { return kc::tune_query(o, q); }
+
+kc_query kc::tune_query(const kc_oriented *o, const kc_query *q)
+{
+    kc_query t = *q;
+    if (q->auto_tune) {
+        t.algo = KC_ALGO_PIVOT;
+        t.part_size = 1;
+        t.gran = (o->m > 0 && o->max_out > 64) ? KC_GRAN_EDGE : KC_GRAN_NODE;
+    }
+    return t;
+}
+
 extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
 {
     if (!o || !q || !r) return fail(KC_ERR_ARG, "kc_count: null argument");
-    if (q->k < 1 || q->k > KC_MAX_K) return fail(KC_ERR_ARG, "kc_count: k=%u outside 1..%d", q->k, KC_MAX_K);
-    if (q->algo > 1 || q->gran > 1) return fail(KC_ERR_ARG, "kc_count: bad algo/gran");
-    uint32_t P = q->part_size;
-    if (P == 0 || P > 64 || (P & (P - 1))) return fail(KC_ERR_ARG, "kc_count: partition size %u", P);
+    int rc = check_query(q);
+    if (rc) return rc;
     hipSetDevice(o->dev);
-    kc_query qq = *q;
-    if (q->auto_tune) {
-        // Measured on gfx950: one lane per candidate row / per DFS partition
-        // (P = 1) is fastest for every size class, and pivoting with
-        // closed-form tails is never slower than the orientation DFS;
-        // edge tasks balance the heavy-tailed degree distributions.
-        qq.algo = KC_ALGO_PIVOT;
-        qq.gran = o->m > 0 ? KC_GRAN_EDGE : KC_GRAN_NODE;
-        qq.part_size = 1;
-    }
-    q = &qq;
-    P = q->part_size;
-    if (q->k <= 2) {
+    const kc_query qq = tune(o, q);
+    std::memset(r, 0, sizeof *r);
+    if (qq.k <= 2) {
         // closed forms: every vertex is a 1-clique, every edge a 2-clique
-        std::memset(r, 0, sizeof *r);
-        r->count = q->k == 1 ? o->n : o->m;
-        r->algo = q->algo;
-        r->gran = q->gran;
-        r->part_size = P;
-        r->tasks = q->gran == KC_GRAN_EDGE ? o->m : o->n;
+        r->count = qq.k == 1 ? o->n : o->m;
+        r->algo = qq.algo;
+        r->gran = qq.gran;
+        r->part_size = qq.part_size;
+        r->tasks = qq.gran == KC_GRAN_EDGE ? o->m : o->n;
         return KC_OK;
     }
-    return count_impl(o, q, r);
+    kc_oriented *om = const_cast<kc_oriented *>(o);
+    std::vector<QueryRun> qs{QueryRun{o, qq}};
+    std::vector<kc_result> res;
+    LaunchStats ls;
+    rc = run_queries(om->scr, om->stream, qs, res, &ls);
+    if (rc) return rc;
+    *r = res[0];
+    r->tasks = ls.tasks;
+    r->alg_bytes = ls.alg_bytes;
+    r->tree_nodes = ls.tree_nodes;
+    r->count_ms = ls.count_ms;
+    r->dominant_ms = ls.dominant_ms;
+    r->dominant_bytes = ls.dominant_bytes;
+    r->dominant_class = ls.dominant_class;
+    for (int c = 0; c < 6; ++c) {
+        r->class_ms[c] = ls.class_ms[c];
+        r->class_bytes[c] = ls.class_bytes[c];
+        r->class_tasks[c] = ls.class_tasks[c];
+        r->class_kernel[c] = ls.class_kernel[c];
+        r->class_max_task_ms[c] = ls.class_max_task_ms[c];
+        r->class_task_ms_sum[c] = ls.class_task_ms_sum[c];
+    }
+    return r->overflow ? fail(KC_ERR_OVERFLOW, "k-clique count exceeds 2^64-1") : KC_OK;
+}
+
+extern "C" int kc_batch_count(kc_oriented *const *gs, const kc_query *qs, size_t nq, kc_result *results,
+                              kc_batch_stats *stats)
+{
+    if ((!gs || !qs || !results) && nq) return fail(KC_ERR_ARG, "kc_batch_count: null argument");
+    std::vector<QueryRun> runs;
+    std::vector<size_t> idx;
+    runs.reserve(nq);
+    int dev = -1;
+    for (size_t i = 0; i < nq; ++i) {
+        if (!gs[i]) return fail(KC_ERR_ARG, "kc_batch_count: null graph %zu", i);
+        int rc = check_query(&qs[i]);
+        if (rc) return rc;
+        if (dev < 0) dev = gs[i]->dev;
+        if (gs[i]->dev != dev) return fail(KC_ERR_ARG, "kc_batch_count: graphs on different devices");
+        const kc_query t = tune(gs[i], &qs[i]);
+        std::memset(&results[i], 0, sizeof(kc_result));
+        results[i].algo = t.algo;
+        results[i].gran = t.gran;
+        results[i].part_size = t.part_size;
+        if (t.k <= 2) {
+            results[i].count = t.k == 1 ? gs[i]->n : gs[i]->m;
+            continue;
+        }
+        runs.push_back(QueryRun{gs[i], t});
+        idx.push_back(i);
+    }
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (runs.empty()) return KC_OK;
+    hipSetDevice(dev);
+    kc_oriented *first = runs[0].o == nullptr ? nullptr : const_cast<kc_oriented *>(runs[0].o);
+    std::vector<kc_result> res;
+    LaunchStats ls;
+    int rc = run_queries(first->batch_scr, first->stream, runs, res, &ls);
+    if (rc) return rc;
+    bool ovf = false;
+    for (size_t j = 0; j < runs.size(); ++j) {
+        results[idx[j]].count = res[j].count;
+        results[idx[j]].overflow = res[j].overflow;
+        ovf |= res[j].overflow != 0;
+    }
+    if (stats) {
+        stats->total_ms = ls.count_ms + ls.prep_ms;
+        stats->count_ms = ls.count_ms;
+        stats->tasks = ls.tasks;
+        stats->queries = runs.size();
+    }
+    return ovf ? fail(KC_ERR_OVERFLOW, "a k-clique count exceeds 2^64-1") : KC_OK;
 }

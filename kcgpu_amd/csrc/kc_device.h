@@ -24,6 +24,7 @@ struct DeviceCtx {
     size_t lds_per_block = 0;
     size_t hbm = 0;
     hipDeviceProp_t prop{};
+    hipStream_t stream = nullptr;  // library stream (default priority), shared by handles
 };
 
 // Select and check the device (gfx950).  Returns KC_OK or KC_ERR_DEVICE.
@@ -43,11 +44,39 @@ struct DevBuf {
 // must fit 32 bits.  in and out may alias.  tmp is scratch owned by caller.
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tmp, hipStream_t s);
 
+// scratch of one launch sequence (grow-only)
+struct Scratch {
+    DevBuf tasks, meta, binom, gscr, size, qdesc;
+};
+
+struct QueryRun {
+    const kc_oriented *o;
+    kc_query q;
+};
+
+struct LaunchStats {
+    double count_ms = 0, prep_ms = 0, dominant_ms = 0;
+    uint64_t tasks = 0, alg_bytes = 0, tree_nodes = 0, dominant_bytes = 0;
+    uint32_t dominant_class = 0;
+    double class_ms[6] = {0}, class_max_task_ms[6] = {0}, class_task_ms_sum[6] = {0};
+    uint64_t class_bytes[6] = {0}, class_tasks[6] = {0};
+    uint32_t class_kernel[6] = {0};
+};
+
+// Count every query of qs in persistent launches on stream s (all graphs on
+// the current device): one launch per (algorithm, size class) bin.
+int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
+                LaunchStats *ls);
+
+// query validation and the autotuner's choice of algo / gran / P for a query
+int check_query(const kc_query *q);
+kc_query tune_query(const kc_oriented *o, const kc_query *q);
+
 }  // namespace kc
 
 struct kc_oriented {
     int dev = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the device's library stream (not owned)
     uint32_t n = 0;
     uint64_t m = 0;
     uint32_t max_out = 0;
@@ -56,7 +85,7 @@ struct kc_oriented {
     double pre_ms = 0;
     kc::DevBuf rowptr, col, src, key;
     // count scratch (grow-only, reused across calls on this handle)
-    kc::DevBuf s_tasks, s_meta, s_binom, s_gscr, s_scan, s_tsz;
+    kc::Scratch scr, batch_scr;
     std::vector<uint32_t> h_rowptr_cache;  // filled lazily by host-side autotune
     ~kc_oriented();
 };

@@ -63,7 +63,13 @@ int device_ctx(DeviceCtx **out)
                 ctx.cu = p.multiProcessorCount;
                 ctx.lds_per_block = p.sharedMemPerBlock;
                 ctx.hbm = p.totalGlobalMem;
-                state = 1;
+                hipSetDevice(d);
+                if (!ctx.stream && hipStreamCreate(&ctx.stream) != hipSuccess) {
+                    state = 2;
+                    why = "hipStreamCreate failed";
+                } else {
+                    state = 1;
+                }
             }
         }
     }
@@ -444,10 +450,7 @@ static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uin
 
 using namespace kc;
 
-kc_oriented::~kc_oriented()
-{
-    if (stream) hipStreamDestroy(stream);
-}
+kc_oriented::~kc_oriented() {}
 
 extern "C" {
 
@@ -475,6 +478,7 @@ int kc_select_device(int ordinal)
     {
         std::lock_guard<std::mutex> lk(g_mu);
         if (hipSetDevice(ordinal) != hipSuccess) return fail(KC_ERR_DEVICE, "hipSetDevice(%d) failed", ordinal);
+        if (g_state == 1 && g_ctx.dev != ordinal) g_ctx.stream = nullptr;  // the old device keeps its stream
         g_state = 0;  // re-read the properties of the selected device
     }
     DeviceCtx *ctx;
@@ -499,7 +503,7 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
         delete o;
         return code;
     };
-    if (hipStreamCreate(&o->stream) != hipSuccess) return bail(fail(KC_ERR_DEVICE, "hipStreamCreate failed"));
+    o->stream = ctx->stream;
     hipStream_t s = o->stream;
     DevBuf frow, fcol, outdeg, tmp;
     if ((rc = frow.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = fcol.ensure(sizeof(uint32_t) * (size_t)nnz + 4)) ||

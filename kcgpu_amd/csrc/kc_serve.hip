@@ -1,0 +1,386 @@
+// kc_serve.hip -- serving loop: oriented graphs resident in HBM, (graph, k)
+// queries admitted by memory footprint and counted by continuous batching.
+//
+// No reference counterpart (the reference counts one graph per process).
+// Design:
+//   * kc_server_add_graph orients a graph on the device and keeps it; it is
+//     refused (KC_ERR_BUDGET) when the resident CSRs plus the orientation's
+//     transient buffers would exceed the HBM budget.
+//   * kc_server_submit only queues the query and returns a ticket.
+//   * `workers` host threads, each with its own HIP stream (default priority)
+//     and grow-only scratch, take every pending query that fits the free
+//     budget (FIFO, at most max_batch) and count them in one run_queries call:
+//     persistent launches draining a queue of (query, task).  Queries that
+//     arrive while a batch is on the GPU go into the next batch, which the
+//     other worker launches on its stream, so consecutive batches overlap on
+//     the device.
+//   * Latency is measured from submit to the moment the worker has the result.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "kc_device.h"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Resident {
+    kc_oriented *o = nullptr;
+    uint64_t bytes = 0;
+    uint32_t pending = 0;
+    bool removed = false;
+};
+
+struct Ticket {
+    uint64_t id;
+    uint32_t gid;
+    kc_query q;
+    uint64_t scratch;  // estimated HBM scratch bytes of this query
+    Clock::time_point t_submit;
+};
+
+struct Done {
+    kc_result r{};
+    double latency_ms = 0;
+    int rc = KC_OK;
+};
+
+}  // namespace
+
+struct kc_server {
+    kc_server_config cfg{};
+    int dev = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    std::unordered_map<uint32_t, Resident> graphs;
+    uint32_t next_gid = 1;
+    uint64_t next_ticket = 1;
+    std::deque<Ticket> pending;
+    std::unordered_map<uint64_t, Done> done;
+    uint64_t resident_bytes = 0, inflight_scratch = 0, held_scratch = 0;
+    uint64_t submitted = 0, completed = 0;             // lifetime, for drain
+    uint64_t st_completed = 0, batches = 0, batched_queries = 0;  // since the last stats reset
+    std::vector<double> lat;
+    Clock::time_point t_first{}, t_last{};
+    bool any = false;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+uint64_t oriented_bytes(uint32_t n, uint64_t m) { return 4ull * (n + 1) + 8ull * m + 4ull * n + 4096; }
+
+// transient device memory of kc_orient: full CSR, out-degrees, peel buffers
+uint64_t orient_transient(const kc_graph *g)
+{
+    const uint64_t n = g->csr.n, nnz = g->csr.col.size();
+    return 4 * (n + 1) + 4 * nnz + 4 * (n + 1) + 13 * n + 4096;
+}
+
+// scratch of one query in a launch: task sizes and codes, plus the HBM bitmap
+// slices when out-degrees exceed the LDS classes
+uint64_t scratch_bytes(const kc::Scratch &c)
+{
+    return c.tasks.bytes + c.meta.bytes + c.binom.bytes + c.gscr.bytes + c.size.bytes + c.qdesc.bytes;
+}
+
+uint64_t query_scratch(const kc_oriented *o, const kc_query &q)
+{
+    const uint64_t ntask = q.gran == KC_GRAN_EDGE ? o->m : o->n;
+    uint64_t b = 12 * ntask + 64 * 1024;
+    if (o->max_out > 1024) b += 2ull << 30;
+    return b;
+}
+
+void worker_main(kc_server *s)
+{
+    hipSetDevice(s->dev);
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess) return;
+    kc::Scratch scr;
+    uint64_t held = 0;  // device bytes this worker's scratch holds between batches
+    for (;;) {
+        std::vector<Ticket> batch;
+        uint64_t batch_scratch = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->stop && s->pending.empty()) break;
+            // admit FIFO while the batch fits next to the resident graphs
+            const uint64_t budget = s->cfg.budget_bytes;
+            while (!s->pending.empty() && batch.size() < s->cfg.max_batch) {
+                const Ticket &t = s->pending.front();
+                const uint64_t need =
+                    s->resident_bytes + s->held_scratch + s->inflight_scratch + batch_scratch + t.scratch;
+                if (!batch.empty() && need > budget) break;
+                batch_scratch += t.scratch;
+                batch.push_back(t);
+                s->pending.pop_front();
+            }
+            s->inflight_scratch += batch_scratch;
+        }
+        std::vector<kc::QueryRun> runs;
+        runs.reserve(batch.size());
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (const Ticket &t : batch) runs.push_back(kc::QueryRun{s->graphs[t.gid].o, t.q});
+        }
+        std::vector<kc_result> res;
+        int rc = kc::run_queries(scr, st, runs, res, nullptr);
+        const auto t_end = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->inflight_scratch -= batch_scratch;
+            const uint64_t now_held = scratch_bytes(scr);
+            s->held_scratch = s->held_scratch - held + now_held;
+            held = now_held;
+            s->batches++;
+            s->batched_queries += batch.size();
+            for (size_t i = 0; i < batch.size(); ++i) {
+                Done d;
+                d.rc = rc;
+                if (rc == KC_OK) {
+                    d.r = res[i];
+                    if (d.r.overflow) d.rc = KC_ERR_OVERFLOW;
+                }
+                d.latency_ms = std::chrono::duration<double, std::milli>(t_end - batch[i].t_submit).count();
+                s->lat.push_back(d.latency_ms);
+                s->done[batch[i].id] = d;
+                Resident &g = s->graphs[batch[i].gid];
+                if (--g.pending == 0 && g.removed) {
+                    s->resident_bytes -= g.bytes;
+                    kc_oriented_free(g.o);
+                    s->graphs.erase(batch[i].gid);
+                }
+            }
+            s->completed += batch.size();
+            s->st_completed += batch.size();
+            s->t_last = t_end;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+    hipStreamDestroy(st);
+}
+
+}  // namespace
+
+using namespace kc;
+
+extern "C" {
+
+int kc_server_create(const kc_server_config *cfg, kc_server **out)
+{
+    if (!out) return fail(KC_ERR_ARG, "kc_server_create: null argument");
+    *out = nullptr;
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    kc_server *s = new kc_server;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.workers == 0) s->cfg.workers = 2;
+    if (s->cfg.workers > 8) s->cfg.workers = 8;
+    if (s->cfg.max_batch == 0) s->cfg.max_batch = 256;
+    if (s->cfg.budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        hipMemGetInfo(&fr, &tot);
+        s->cfg.budget_bytes = (uint64_t)(fr * 0.9);
+    }
+    s->dev = ctx->dev;
+    for (uint32_t i = 0; i < s->cfg.workers; ++i) s->workers.emplace_back(worker_main, s);
+    *out = s;
+    return KC_OK;
+}
+
+int kc_server_add_graph(kc_server *s, const kc_graph *g, int orient_mode, uint32_t *gid)
+{
+    if (!s || !g || !gid) return fail(KC_ERR_ARG, "kc_server_add_graph: null argument");
+    const uint64_t foot = oriented_bytes(g->csr.n, g->csr.col.size() / 2);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const uint64_t need =
+            s->resident_bytes + s->held_scratch + s->inflight_scratch + foot + orient_transient(g);
+        if (need > s->cfg.budget_bytes)
+            return fail(KC_ERR_BUDGET, "graph needs %llu bytes, %llu of %llu in use",
+                        (unsigned long long)(foot + orient_transient(g)),
+                        (unsigned long long)(s->resident_bytes + s->inflight_scratch),
+                        (unsigned long long)s->cfg.budget_bytes);
+        s->resident_bytes += foot;  // reserve before the upload
+    }
+    kc_oriented *o = nullptr;
+    int rc = kc_orient(g, orient_mode, &o);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc) {
+        s->resident_bytes -= foot;
+        return rc;
+    }
+    Resident r;
+    r.o = o;
+    r.bytes = foot;
+    *gid = s->next_gid++;
+    s->graphs[*gid] = r;
+    return KC_OK;
+}
+
+int kc_server_remove_graph(kc_server *s, uint32_t gid)
+{
+    if (!s) return fail(KC_ERR_ARG, "kc_server_remove_graph: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->graphs.find(gid);
+    if (it == s->graphs.end() || it->second.removed) return fail(KC_ERR_NOTFOUND, "no graph %u", gid);
+    if (it->second.pending == 0) {
+        s->resident_bytes -= it->second.bytes;
+        kc_oriented_free(it->second.o);
+        s->graphs.erase(it);
+    } else {
+        it->second.removed = true;  // freed by the worker after its last query
+    }
+    return KC_OK;
+}
+
+int kc_server_submit(kc_server *s, uint32_t gid, const kc_query *q, uint64_t *ticket)
+{
+    if (!s || !q || !ticket) return fail(KC_ERR_ARG, "kc_server_submit: null argument");
+    int rc = check_query(q);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->graphs.find(gid);
+    if (it == s->graphs.end() || it->second.removed) return fail(KC_ERR_NOTFOUND, "no graph %u", gid);
+    Ticket t;
+    t.id = s->next_ticket++;
+    t.gid = gid;
+    t.q = tune_query(it->second.o, q);
+    t.t_submit = Clock::now();
+    if (t.q.k <= 2) {  // closed forms, no device work
+        Done d;
+        d.r.count = t.q.k == 1 ? it->second.o->n : it->second.o->m;
+        d.r.algo = t.q.algo;
+        d.r.gran = t.q.gran;
+        d.r.part_size = t.q.part_size;
+        s->done[t.id] = d;
+        s->lat.push_back(0.0);
+        s->submitted++;
+        s->completed++;
+        s->st_completed++;
+        *ticket = t.id;
+        return KC_OK;
+    }
+    t.scratch = query_scratch(it->second.o, t.q);
+    if (s->resident_bytes + t.scratch > s->cfg.budget_bytes)
+        return fail(KC_ERR_BUDGET, "query scratch %llu bytes does not fit next to the resident graphs",
+                    (unsigned long long)t.scratch);
+    it->second.pending++;
+    if (!s->any) {
+        s->t_first = t.t_submit;
+        s->any = true;
+    }
+    s->pending.push_back(t);
+    s->submitted++;
+    *ticket = t.id;
+    s->cv_work.notify_one();
+    return KC_OK;
+}
+
+int kc_server_poll(kc_server *s, uint64_t ticket, kc_result *r, int *done)
+{
+    if (!s || !r || !done) return fail(KC_ERR_ARG, "kc_server_poll: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->done.find(ticket);
+    if (it == s->done.end()) {
+        if (ticket == 0 || ticket >= s->next_ticket) return fail(KC_ERR_NOTFOUND, "no ticket %llu", (unsigned long long)ticket);
+        *done = 0;
+        return KC_OK;
+    }
+    *done = 1;
+    *r = it->second.r;
+    const int rc = it->second.rc;
+    s->done.erase(it);
+    return rc == KC_OK ? KC_OK : fail(rc, "query %llu failed", (unsigned long long)ticket);
+}
+
+int kc_server_wait(kc_server *s, uint64_t ticket, kc_result *r)
+{
+    if (!s || !r) return fail(KC_ERR_ARG, "kc_server_wait: null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket == 0 || ticket >= s->next_ticket) return fail(KC_ERR_NOTFOUND, "no ticket %llu", (unsigned long long)ticket);
+    s->cv_done.wait(lk, [&] { return s->done.count(ticket) != 0; });
+    auto it = s->done.find(ticket);
+    *r = it->second.r;
+    const int rc = it->second.rc;
+    s->done.erase(it);
+    return rc == KC_OK ? KC_OK : fail(rc, "query %llu failed", (unsigned long long)ticket);
+}
+
+int kc_server_drain(kc_server *s)
+{
+    if (!s) return fail(KC_ERR_ARG, "kc_server_drain: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->completed == s->submitted; });
+    return KC_OK;
+}
+
+int kc_server_get_stats(kc_server *s, kc_server_stats *st)
+{
+    if (!s || !st) return fail(KC_ERR_ARG, "kc_server_get_stats: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(st, 0, sizeof *st);
+    st->submitted = s->submitted;
+    st->completed = s->st_completed;
+    st->batches = s->batches;
+    st->mean_batch = s->batches ? (double)s->batched_queries / s->batches : 0.0;
+    st->graphs = s->graphs.size();
+    st->resident_bytes = s->resident_bytes;
+    std::vector<double> l = s->lat;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+        st->p50_ms = pct(0.50);
+        st->p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : l) sum += x;
+        st->mean_ms = sum / l.size();
+    }
+    if (s->any && s->st_completed) {
+        const double span = std::chrono::duration<double>(s->t_last - s->t_first).count();
+        st->queries_per_s = span > 0 ? s->st_completed / span : 0.0;
+    }
+    st->held_scratch_bytes = s->held_scratch;
+    return KC_OK;
+}
+
+int kc_server_reset_stats(kc_server *s)
+{
+    if (!s) return fail(KC_ERR_ARG, "kc_server_reset_stats: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat.clear();
+    s->st_completed = 0;
+    s->batches = 0;
+    s->batched_queries = 0;
+    s->any = false;
+    return KC_OK;
+}
+
+void kc_server_destroy(kc_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (auto &kv : s->graphs) kc_oriented_free(kv.second.o);
+    delete s;
+}
+
+}  // extern "C"

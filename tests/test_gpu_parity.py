@@ -178,3 +178,96 @@ def test_empty_and_tiny_graphs():
                 for algo in ALGOS:
                     for gran in GRANS:
                         assert Og.count(k, algo, gran).count == exp
+
+
+def _small_mix(rng, count):
+    """uneven small graphs (BA and ER), as in the batched workload"""
+    out = []
+    for i in range(count):
+        n = int(rng.integers(50, 3000))
+        if i % 2 == 0:
+            G = K.Graph.generate(K.GEN_BA, seed=1000 + i, n=n, attach=int(rng.integers(2, 12)))
+        else:
+            m = min(n * int(rng.integers(2, 20)) // 2, n * (n - 1) // 4)
+            G = K.Graph.generate(K.GEN_ER, seed=1000 + i, n=n, m=m)
+        out.append(G)
+    return out
+
+
+def test_batch_equals_alone():
+    rng = np.random.default_rng(7)
+    graphs = _small_mix(rng, 40)
+    graphs.append(_planted(3000, 12000, [90, 200, 400], seed=3))  # bins above 64 vertices
+    ors = [G.orient("degen" if i % 3 else "degree") for i, G in enumerate(graphs)]
+    pairs = []
+    for i, Og in enumerate(ors):
+        for _ in range(2):  # the same graph twice, with different queries
+            q = dict(k=int(rng.integers(1, 9)), algo=ALGOS[int(rng.integers(0, 2))],
+                     gran=GRANS[int(rng.integers(0, 2))], part_size=int(2 ** rng.integers(0, 7)))
+            pairs.append((Og, q))
+    pairs.append((ors[-1], dict(k=6, algo="pivot", gran="edge", auto_tune=True)))
+    alone = [Og.count(**q).count for Og, q in pairs]
+    mixed, st = K.batch_count(pairs)
+    print("batch", st, flush=True)
+    assert [r.count for r in mixed] == alone
+    # a second batch on the same scratch, smaller, still exact
+    mixed2, _ = K.batch_count(pairs[:7])
+    assert [r.count for r in mixed2] == alone[:7]
+
+
+def test_batch_overflow_is_per_query():
+    n = 70
+    a, b = np.triu_indices(n, 1)
+    K70 = K.Graph.from_edges(a.astype(np.uint32), b.astype(np.uint32), n).orient("degree")
+    small = K.Graph.generate(K.GEN_ER, seed=3, n=500, m=3000).orient("degen")
+    pairs = [(K70, dict(k=30, algo="pivot", gran="node")), (small, dict(k=3, algo="pivot", gran="edge")),
+             (K70, dict(k=10, algo="pivot", gran="edge"))]
+    res, _ = K.batch_count(pairs, check=False)
+    assert res[0].overflow and not res[1].overflow and not res[2].overflow
+    assert res[1].count == small.count(3, "orient", "edge").count
+    assert res[2].count == comb(70, 10)
+
+
+def test_serving_results_equal_alone():
+    import threading
+    rng = np.random.default_rng(11)
+    graphs = _small_mix(rng, 12) + [_planted(2500, 9000, [70, 150], seed=4)]
+    srv = K.Server(workers=2, max_batch=16)
+    try:
+        gids = [srv.add_graph(G, "degen" if i % 2 else "degree") for i, G in enumerate(graphs)]
+        ors = {gid: G.orient("degen" if i % 2 else "degree") for i, (gid, G) in enumerate(zip(gids, graphs))}
+        qs = [(gids[int(rng.integers(0, len(gids)))], int(rng.integers(1, 10)), ALGOS[int(rng.integers(0, 2))],
+               GRANS[int(rng.integers(0, 2))]) for _ in range(120)]
+        tickets = [None] * len(qs)
+
+        def submit(lo, hi):  # several client threads, queries arrive while others run
+            for i in range(lo, hi):
+                g, k, a, gr = qs[i]
+                tickets[i] = srv.submit(g, k, a, gr)
+
+        th = [threading.Thread(target=submit, args=(i * 30, (i + 1) * 30)) for i in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        mixed = [srv.wait(t).count for t in tickets]
+        alone = [ors[g].count(k, a, gr).count for g, k, a, gr in qs]
+        assert mixed == alone
+        st = srv.stats()
+        print("serving", st, flush=True)
+        assert st["completed"] == len(qs) and st["batches"] >= 1
+        # a graph larger than the budget left is refused
+        big = K.Graph.generate(K.GEN_ER, seed=5, n=200000, m=2000000)
+        tiny = K.Server(budget_bytes=8 << 20, workers=1)
+        try:
+            with pytest.raises(K.KCError) as ei:
+                tiny.add_graph(big)
+            assert ei.value.code == K.KC_ERR_BUDGET
+        finally:
+            tiny.close()
+        srv.remove_graph(gids[0])
+        with pytest.raises(K.KCError) as ei:
+            srv.submit(gids[0], 4)
+        assert ei.value.code == K.KC_ERR_NOTFOUND
+    finally:
+        srv.close()
