@@ -73,6 +73,27 @@ int kc_load(const char *path, unsigned flags, kc_graph **out);
 int kc_read_edges_reference(const char *path, uint32_t **src, uint32_t **dst, size_t *count);
 void kc_free_buffer(void *p);
 
+/* ---- format converters (-m mtx-bel | tsv-bel | tsv-mtx | bel-mtx | txt-bel) */
+#define KC_CONV_MTX_BEL 0  /* MTX -> BEL: symmetric entries mirrored, sorted, loops kept once */
+#define KC_CONV_TSV_BEL 1  /* "dst src weight" triples -> BEL in file order */
+#define KC_CONV_TSV_MTX 2  /* "dst src weight" triples -> general MTX "src dst weight" */
+#define KC_CONV_BEL_MTX 3  /* BEL -> general MTX "src dst weight" */
+#define KC_CONV_TXT_BEL 4  /* "a b" lines -> BEL of both directions, sorted, deduplicated, no loops */
+
+/* kc_convert flags */
+#define KC_CONV_REFERENCE_FLAGS 1u  /* mtx-bel: every '%' line re-reads the symmetric/pattern
+                                       flags, as the reference does (a later comment clears them) */
+
+/*
+ * Convert between the on-disk formats.  Replaces graph::MtB_Writer
+ * (include/FIleReader.cuh:480-933) as dispatched by src/main.cu:44-71
+ * (-g source, -r destination).  Output bytes equal the reference's for
+ * inputs on which the reference is well defined; where it is not (no size
+ * line, an empty input, negative or >32-bit ids, an empty line in a .txt)
+ * this returns KC_ERR_FORMAT, or skips the empty line.
+ */
+int kc_convert(int kind, const char *src_path, const char *dst_path, unsigned flags);
+
 /* Build from an edge list (both arrays of length nedges; n = 0: max id + 1). */
 int kc_graph_from_edges(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst,
                         kc_graph **out);
@@ -139,6 +160,18 @@ int kc_oriented_get_info(const kc_oriented *o, kc_oriented_info *info);
  * (degree or peel round, key[n]) to the host; any pointer may be NULL. */
 int kc_oriented_export(const kc_oriented *o, uint32_t *rowptr, uint32_t *col, uint32_t *key);
 void kc_oriented_free(kc_oriented *o);
+
+/*
+ * Binary oriented-CSR cache: save an oriented graph and load it back onto
+ * the device without loading, cleaning or orienting again.  The file is
+ * little-endian: "KCOCSR1\0", u32 n, u32 mode, u64 m, u32 max_out_degree,
+ * u32 max_core, u64 FNV-1a 64 of the three arrays, then rowptr[n+1], col[m],
+ * key[n] (u32).  Loading checks the header, the checksum and that the CSR is
+ * well formed (KC_ERR_FORMAT otherwise).  No reference counterpart: it cuts
+ * src/main.cu:73-235 (read, build, orient) for repeated runs and uploads.
+ */
+int kc_oriented_save(const kc_oriented *o, const char *path);
+int kc_oriented_load(const char *path, kc_oriented **out);
 
 /* ---- counting ---------------------------------------------------------- */
 #define KC_ALGO_ORIENT 0  /* -q o... */

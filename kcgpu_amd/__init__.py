@@ -127,6 +127,9 @@ SIGNATURES = {
     "kc_oriented_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(OrientedInfo)]),
     "kc_oriented_export": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
     "kc_oriented_free": (None, [_vp]),
+    "kc_oriented_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "kc_oriented_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "kc_convert": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
     "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
@@ -258,6 +261,16 @@ def read_edges_reference(path):
     return src, dst
 
 
+CONVERSIONS = {"mtx-bel": 0, "tsv-bel": 1, "tsv-mtx": 2, "bel-mtx": 3, "txt-bel": 4}
+KC_CONV_REFERENCE_FLAGS = 1
+
+
+def convert(kind, src, dst, reference_flags=False):
+    """-m mtx-bel | tsv-bel | tsv-mtx | bel-mtx | txt-bel: write dst from src."""
+    _check(lib().kc_convert(CONVERSIONS[kind] if isinstance(kind, str) else int(kind), os.fsencode(src),
+                            os.fsencode(dst), KC_CONV_REFERENCE_FLAGS if reference_flags else 0))
+
+
 def parse_spec(spec):
     """-q grammar [o|p]<P>[b|n|h] -> dict(algo, part_size, binary)."""
     q = Query()
@@ -297,6 +310,17 @@ class Oriented:
         _check(lib().kc_oriented_export(self._h, rp.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p),
                                         key.ctypes.data_as(_u32p)))
         return rp, col[: inf["m"]], key[: inf["n"]]
+
+    def save(self, path):
+        """Write the binary oriented-CSR cache file."""
+        _check(lib().kc_oriented_save(self._h, os.fsencode(path)))
+
+    @staticmethod
+    def load(path):
+        """Load a cache file written by save() straight into HBM."""
+        h = ctypes.c_void_p()
+        _check(lib().kc_oriented_load(os.fsencode(path), ctypes.byref(h)))
+        return Oriented(h.value)
 
     def count(self, k, algo="pivot", gran="node", part_size=8, binary=True, auto_tune=False, check=True):
         q = Query(int(k), ALGO[algo] if isinstance(algo, str) else int(algo),

@@ -22,7 +22,7 @@ BIN = os.path.join(HERE, "bin", "kcgpu")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-CPP = ["kc_io.cpp", "kc_gen.cpp", "kc_api.cpp"]
+CPP = ["kc_io.cpp", "kc_gen.cpp", "kc_api.cpp", "kc_convert.cpp"]
 HIP = ["kc_prep.hip", "kc_count.hip", "kc_serve.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter"]

@@ -1,12 +1,16 @@
 // kc_main.cpp -- drop-in command line for the reference's `-m kc` path
 // (src/main.cu:35-588, flags from include/Config.h:317-363).
 //
-//   kcgpu -g graph.{mtx,tsv,txt,bel} [-m kc|kcore] [-o degree|degen] [-k K]
+//   kcgpu -g graph.{mtx,tsv,txt,bel,kco} [-m kc|kcore] [-o degree|degen] [-k K]
 //         [-p node|edge] [-q {o|p}<P>[b|n]] [-d dev] [-w x] [-s x] [-x x] [-a ...]
+//   kcgpu -m mtx-bel|tsv-bel|tsv-mtx|bel-mtx|txt-bel -g in -r out   (src/main.cu:44-71)
+//   kcgpu -m cache -g graph -o degree|degen -r out.kco              (oriented-CSR cache)
 //
 // Every reference flag takes an argument (getopt "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:");
-// -w, -s, -x, -a, -e, -v, -r, -t are accepted for compatibility and do not
-// change the result.  Output keeps the reference's lines (Read graph time,
+// -w, -s, -x, -a, -e, -v, -t are accepted for compatibility and do not
+// change the result; -r names the output of a conversion.  A .kco input is
+// an oriented-CSR cache file: it is counted with the orientation it was
+// saved with, without reading, cleaning or orienting again.  Output keeps the reference's lines (Read graph time,
 // Preprocess time, Max Core, the Level/Nodes/Counter line, count time and the
 // MOHA ... teps line) with the true count in place of the reference's level
 // value, plus a `kcliques=` line and one JSON metrics line.
@@ -50,7 +54,10 @@ void usage()
     std::fprintf(stderr,
                  "\nUsage:  kcgpu [options]\n"
                  "    -g <graph file>   .mtx, .tsv, .txt or .bel\n"
-                 "    -m <task>         kc (k-clique count, default) or kcore (max core)\n"
+                 "    -m <task>         kc (k-clique count, default) or kcore (max core);\n"
+                 "                      mtx-bel, tsv-bel, tsv-mtx, bel-mtx, txt-bel convert -g into -r;\n"
+                 "                      cache writes the oriented CSR of -g (-o) into -r (.kco)\n"
+                 "    -r <file>         output of a conversion or of -m cache\n"
                  "    -o <orient>       degree (default) or degen\n"
                  "    -k <k>            clique size (default 5)\n"
                  "    -p <unit>         node (default) or edge\n"
@@ -58,7 +65,7 @@ void usage()
                  "                      P in 1,2,4,...,64, bitmap (b) or index lists (n); default o8b\n"
                  "    -d <device>       HIP device ordinal (default 0)\n"
                  "    -t auto           let the autotuner pick algorithm, granularity and P\n"
-                 "    -w/-s/-x/-a/-e/-v/-r <arg>  accepted for compatibility\n"
+                 "    -w/-s/-x/-a/-e/-v <arg>  accepted for compatibility\n"
                  "    -h <any>          help\n\n");
 }
 
@@ -72,7 +79,7 @@ int die(int rc, const char *what)
 
 int main(int argc, char **argv)
 {
-    const char *graph = nullptr;
+    const char *graph = nullptr, *dst = nullptr;
     std::string task = "kc", orient = "degree", by = "node", spec = "o8b";
     int k = 5, dev = 0;
     bool autotune = false;
@@ -87,7 +94,8 @@ int main(int argc, char **argv)
         case 'p': by = optarg; break;
         case 'q': spec = optarg; break;
         case 't': autotune = std::strcmp(optarg, "auto") == 0; break;
-        case 'r': case 'x': case 'a': case 'v': case 's': case 'e': case 'w': break;
+        case 'r': dst = optarg; break;
+        case 'x': case 'a': case 'v': case 's': case 'e': case 'w': break;
         case 'h': usage(); return 0;
         default: usage(); return 1;
         }
@@ -96,8 +104,26 @@ int main(int argc, char **argv)
         usage();
         return 1;
     }
-    if (task != "kc" && task != "kcore") {
-        std::fprintf(stderr, "error: -m %s is not part of this build (kc, kcore)\n", task.c_str());
+    static const char *conv[] = {"mtx-bel", "tsv-bel", "tsv-mtx", "bel-mtx", "txt-bel"};
+    for (int i = 0; i < 5; ++i)
+        if (task == conv[i]) {
+            if (!dst) {
+                std::fprintf(stderr, "error: -m %s needs -r <output file>\n", task.c_str());
+                return 1;
+            }
+            double tc0 = now();
+            int crc = kc_convert(i, graph, dst, 0);
+            if (crc) return die(crc, task.c_str());
+            stamp("Converted %s -> %s (%s) in %f s", graph, dst, task.c_str(), now() - tc0);
+            return 0;
+        }
+    if (task != "kc" && task != "kcore" && task != "cache") {
+        std::fprintf(stderr, "error: -m %s is not part of this build (kc, kcore, cache, mtx-bel, tsv-bel, "
+                             "tsv-mtx, bel-mtx, txt-bel)\n", task.c_str());
+        return 1;
+    }
+    if (task == "cache" && !dst) {
+        std::fprintf(stderr, "error: -m cache needs -r <output .kco file>\n");
         return 1;
     }
     int omode;
@@ -128,20 +154,31 @@ int main(int argc, char **argv)
 
     double t0 = now();
     kc_graph *g = nullptr;
-    if ((rc = kc_load(graph, 0, &g))) return die(rc, "load");
-    kc_graph_info gi;
-    kc_graph_get_info(g, &gi);
-    stamp("Read graph time: %f s", now() - t0);
-    std::printf("    n = %u, undirected edges = %" PRIu64 ", max degree = %u\n", gi.n, gi.nnz / 2, gi.max_degree);
+    kc_oriented *o = nullptr;
+    const size_t glen = std::strlen(graph);
+    const bool cached = glen > 4 && std::strcmp(graph + glen - 4, ".kco") == 0;
+    if (cached) {
+        if ((rc = kc_oriented_load(graph, &o))) return die(rc, "load");
+        stamp("Read graph time: %f s", now() - t0);
+    } else {
+        if ((rc = kc_load(graph, 0, &g))) return die(rc, "load");
+        kc_graph_info gi;
+        kc_graph_get_info(g, &gi);
+        stamp("Read graph time: %f s", now() - t0);
+        std::printf("    n = %u, undirected edges = %" PRIu64 ", max degree = %u\n", gi.n, gi.nnz / 2, gi.max_degree);
+    }
 
     double t1 = now();
-    kc_oriented *o = nullptr;
-    if ((rc = kc_orient(g, omode, &o))) return die(rc, "orient");
+    if (!cached && (rc = kc_orient(g, omode, &o))) return die(rc, "orient");
     kc_oriented_info oi;
     kc_oriented_get_info(o, &oi);
     if (omode == KC_ORIENT_DEGEN) std::printf("Max Core = %u\n", oi.max_core);
     stamp("Preprocess time: %f s", now() - t1);
-    if (task == "kcore") {
+    if (task == "cache") {
+        if ((rc = kc_oriented_save(o, dst))) return die(rc, "cache");
+        stamp("Wrote oriented CSR to %s", dst);
+    }
+    if (task == "kcore" || task == "cache") {
         kc_oriented_free(o);
         kc_graph_free(g);
         return 0;

@@ -485,7 +485,6 @@ template <int S>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
                               uint64_t &nodes, uint32_t &ovf)
 {
-    constexpr int NB = (S == 1) ? 7 : (S == 2) ? 8 : 9;  // bits of (score + 1) <= 64*S
     const uint32_t lane = (uint32_t)lane_id();
     const uint32_t W = (R + 63) / 64;
     uint64_t row[S][S];
@@ -525,27 +524,24 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
             } else if (h + 1 == k) {
                 add_checked(acc, (uint64_t)p + cx, ovf);
             } else if (h + p + cx >= k) {
-                uint32_t sv[S], lsum = 0;
+                // scores |X & row_j| of the lane's member rows; key packs
+                // (score + 1, 0xFFFF - j), so the wave max names the smallest j of
+                // the largest score
+                uint32_t key = 0, lsum = 0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const bool mem = (X[s] >> lane) & 1ull;
                     uint32_t v = 0;
 #pragma unroll
                     for (int w = 0; w < S; ++w) v += (uint32_t)__popcll(X[w] & row[s][w]);
-                    sv[s] = mem ? v + 1 : 0;
-                    lsum += mem ? v : 0;
-                }
-                uint32_t best = 0;
-#pragma unroll
-                for (int bt = NB; bt >= 0; --bt) {
-                    const uint32_t c = best | (1u << bt);
-                    bool any = false;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) any |= sv[s] >= c;
-                    if (__ballot(any)) best = c;
+                    if ((X[s] >> lane) & 1ull) {
+                        lsum += v;
+                        const uint32_t kk = ((v + 1) << 16) | (0xFFFFu - (64u * s + lane));
+                        key = kk > key ? kk : key;
+                    }
                 }
                 if (h + 3 >= k) {
-                    const uint32_t e2 = ballot_sum(lsum, NB + 3);  // twice the edges of G[X]
+                    // twice the edges of G[X]
+                    const uint32_t e2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(lsum));
                     const uint64_t pp = p;
                     if (h + 2 == k) {
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2, ovf);
@@ -554,49 +550,49 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                     } else {
                         // h = k-3: C(p,3) + C(p,2)|X| + p|E(X)| + triangles(X); each
                         // triangle i < j < l is counted once, at its smallest vertex
+                        // rows i live in slot i/64 of lane i%64: fetched with v_readlane
+                        // from a register chosen at compile time (wi is unrolled)
                         uint32_t tri = 0;
+                        const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+#pragma unroll
                         for (int wi = 0; wi < S; ++wi) {
                             uint64_t bits = X[wi];
                             while (bits) {
-                                const uint32_t i = 64u * wi + (uint32_t)__builtin_ctzll(bits);
+                                const uint32_t il = (uint32_t)__builtin_ctzll(bits);
                                 bits &= bits - 1;
                                 uint64_t Y[S];
-                                bool any = false;
+                                uint32_t ny = 0;
 #pragma unroll
                                 for (int w = 0; w < S; ++w) {
-                                    uint64_t above = (w < wi) ? 0ull : (w > wi) ? ~0ull
-                                                   : ((i & 63) == 63 ? 0ull : (~0ull << ((i & 63) + 1)));
-                                    Y[w] = X[w] & lds_row_word(rows, W, i, w) & above;
-                                    any |= Y[w] != 0;
+                                    if (w < wi) {
+                                        Y[w] = 0;
+                                        continue;
+                                    }
+                                    const uint64_t above = (w > wi) ? ~0ull : (il == 63 ? 0ull : (~0ull << (il + 1)));
+                                    Y[w] = X[w] & rdlane64(row[wi][w], il) & above;
+                                    ny += (uint32_t)__popcll(Y[w]);
                                 }
-                                if (!any) continue;
+                                if (ny < 2) continue;
 #pragma unroll
-                                for (int sl = 0; sl < S; ++sl) {
+                                for (int sl = wi; sl < S; ++sl) {
                                     if (!((Y[sl] >> lane) & 1ull)) continue;
                                     uint32_t v = 0;
 #pragma unroll
-                                    for (int w = 0; w < S; ++w) {
-                                        uint64_t above = (w < sl) ? 0ull : (w > sl) ? ~0ull
-                                                       : (lane == 63 ? 0ull : (~0ull << (lane + 1)));
-                                        v += (uint32_t)__popcll(Y[w] & row[sl][w] & above);
-                                    }
+                                    for (int w = sl; w < S; ++w)
+                                        v += (uint32_t)__popcll(Y[w] & row[sl][w] & (w > sl ? ~0ull : lane_above));
                                     tri += v;
                                 }
                             }
                         }
-                        tri = wave_sum32(tri);
+                        tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) * (pp >= 2 ? pp - 2 : 0) / 6, ovf);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2 * cx, ovf);
                         add_checked(acc, pp * (e2 / 2), ovf);
                         add_checked(acc, tri, ovf);
                     }
-                } else if (h + p + best >= k) {  // best = max score + 1
-                    uint32_t pv = 0;
-#pragma unroll
-                    for (int s = S - 1; s >= 0; --s) {
-                        const uint64_t bal = __ballot(sv[s] == best);
-                        if (bal) pv = 64u * s + (uint32_t)__builtin_ctzll(bal);
-                    }
+                } else if ((((key = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32(key))) >> 16) +
+                            h + p) >= k) {  // (key >> 16) = max score + 1 bounds the clique inside X
+                    const uint32_t pv = 0xFFFFu - (key & 0xFFFFu);
                     ++t;
 #pragma unroll
                     for (int w = 0; w < S; ++w) {

@@ -6,7 +6,9 @@
 //   getNodeDegree_kernel            kcore/kcore.cuh:50-59
 //   findKcoreIncremental_async      kcore/kcore.cuh:442-584 (host loop, a D->H
 //                                   count read per round) -> one cooperative
-//                                   kernel with grid barriers
+//                                   kernel with grid barriers, or, when the
+//                                   cooperative launch is refused, steps of
+//                                   five kernels queued in batches of 32
 //   init (keep flags)               include/main_support.cuh:54-128
 //   CUBSelect x2                    include/cub_wrappers.cuh:121-179
 //   warp_detect_deleted_edges       include/main_support.cuh:133-167
@@ -18,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -397,6 +400,132 @@ __global__ __launch_bounds__(256) void k_peel(PeelArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same peel without a cooperative launch, for processes that cannot get
+// one (the device's cooperative launch resource is taken by another process).
+// A one-thread control kernel keeps the state machine that k_peel keeps in
+// registers; the host enqueues steps of five kernels in batches and reads the
+// phase once per batch.  Kernels of a step that has nothing to do return at
+// once, so a batch that runs past the end is cheap.
+// ---------------------------------------------------------------------------
+struct PeelCtl {
+    uint32_t phase;  // 0 gather, 1 round, 2 done
+    uint32_t level, round, c;
+    int64_t todo;
+};
+
+__global__ void k_peel_ctl_init(PeelCtl *ctl, uint32_t *cnt, uint32_t n)
+{
+    ctl->phase = n ? 0 : 2;
+    ctl->level = ctl->round = ctl->c = 0;
+    ctl->todo = n;
+    cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_peel_gather(PeelArgs a, const PeelCtl *ctl)
+{
+    if (ctl->phase != 0) return;
+    const uint32_t c = ctl->c, level = ctl->level;
+    uint32_t *q = c ? a.q1 : a.q0;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t wave = tid >> 6, nwave = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t base = wave * 64; base < a.n; base += nwave * 64) {
+        uint64_t v = base + lane;
+        bool take = v < a.n && a.state[v] == 0 && a.deg[v] == (int)level;
+        wave_append(take, (uint32_t)v, q, &a.cnt[c]);
+    }
+}
+
+// after a gather: an empty level moves on, a frontier starts a round, the
+// frontier that empties the graph ends the peel unassigned
+__global__ void k_peel_ctl_gathered(PeelCtl *ctl, uint32_t *cnt, uint32_t maxdeg)
+{
+    if (ctl->phase != 0) return;
+    const uint32_t c = ctl->c, nc = cnt[c];
+    if (nc == 0) {
+        if (++ctl->level > maxdeg + 1) {
+            cnt[3] = 1;
+            ctl->phase = 2;
+        }
+        return;  // cnt[c] is still 0 for the next gather
+    }
+    ctl->todo -= nc;
+    if (ctl->todo == 0) {
+        cnt[2] = ctl->level;
+        ctl->phase = 2;
+        return;
+    }
+    cnt[c ^ 1] = 0;
+    ctl->phase = 1;
+}
+
+__global__ __launch_bounds__(256) void k_peel_assign(PeelArgs a, const PeelCtl *ctl)
+{
+    if (ctl->phase != 1) return;
+    const uint32_t c = ctl->c, nc = a.cnt[c], round = ctl->round;
+    const uint32_t *q = c ? a.q1 : a.q0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = q[i];
+        a.state[v] = 1;
+        a.prio[v] = round;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_peel_decrement(PeelArgs a, const PeelCtl *ctl)
+{
+    if (ctl->phase != 1) return;
+    const uint32_t c = ctl->c, nc = a.cnt[c], level = ctl->level;
+    const uint32_t *q = c ? a.q1 : a.q0;
+    uint32_t *qn = c ? a.q0 : a.q1;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t wave = tid >> 6, nwave = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t i = wave; i < nc; i += nwave) {
+        uint32_t v = q[i];
+        uint32_t b = a.rowptr[v], e = a.rowptr[v + 1];
+        for (uint32_t j = b; j < e; j += 64) {
+            bool take = false;
+            uint32_t x = 0;
+            if (j + lane < e) {
+                x = a.col[j + lane];
+                if (a.state[x] == 0) {
+                    int old = atomicSub(&a.deg[x], 1);
+                    take = (old == (int)level + 1);
+                }
+            }
+            wave_append(take, x, qn, &a.cnt[c ^ 1]);
+        }
+    }
+}
+
+// after a round: the next frontier continues the level, an empty one ends it
+__global__ void k_peel_ctl_round(PeelCtl *ctl, uint32_t *cnt, uint32_t maxdeg)
+{
+    if (ctl->phase != 1) return;
+    const uint32_t c = ctl->c ^ 1;
+    ctl->c = c;
+    ctl->round++;
+    const uint32_t nc = cnt[c];
+    if (nc > 0) {
+        ctl->todo -= nc;
+        if (ctl->todo == 0) {
+            cnt[2] = ctl->level;
+            ctl->phase = 2;
+            return;
+        }
+        cnt[c ^ 1] = 0;
+        return;
+    }
+    if (++ctl->level > maxdeg + 1) {
+        cnt[3] = 1;
+        ctl->phase = 2;
+        return;
+    }
+    cnt[c] = 0;
+    ctl->phase = 0;
+}
+
 __global__ void k_peel_init(uint32_t n, const uint32_t *rowptr, int *deg, uint32_t *prio, uint8_t *state,
                             uint32_t nnz)
 {
@@ -437,7 +566,36 @@ static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uin
     a.q1 = q1.as<uint32_t>();
     a.cnt = cnt.as<uint32_t>();
     void *args[] = {&a};
-    KC_HIP(hipLaunchCooperativeKernel((const void *)k_peel, dim3(grid), dim3(256), args, 0, s));
+    const char *mode = std::getenv("KC_PEEL_MODE");  // "host": always take the stepped peel
+    bool coop = !(mode && std::strcmp(mode, "host") == 0);
+    if (coop && hipLaunchCooperativeKernel((const void *)k_peel, dim3(grid), dim3(256), args, 0, s) != hipSuccess) {
+        (void)hipGetLastError();  // the launch was refused, nothing ran: step the peel from the host
+        coop = false;
+    }
+    if (!coop) {
+        DevBuf ctlb;
+        if ((rc = ctlb.ensure(sizeof(PeelCtl)))) return rc;
+        PeelCtl *ctl = ctlb.as<PeelCtl>();
+        hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, a.cnt, n);
+        const int gs = std::max(1, std::min(gi, ctx->cu * 4));
+        // every step peels a vertex or moves up a level: n + maxdeg + 2 steps end it
+        const uint64_t max_steps = (uint64_t)n + maxdeg + 2;
+        uint64_t steps = 0;
+        uint32_t phase = 0;
+        while (phase != 2) {
+            if (steps > max_steps) return fail(KC_ERR_DEVICE, "peel did not converge (internal error)");
+            for (int i = 0; i < 32; ++i, ++steps) {
+                hipLaunchKernelGGL(k_peel_gather, dim3(gs), dim3(256), 0, s, a, ctl);
+                hipLaunchKernelGGL(k_peel_ctl_gathered, dim3(1), dim3(1), 0, s, ctl, a.cnt, maxdeg);
+                hipLaunchKernelGGL(k_peel_assign, dim3(gs), dim3(256), 0, s, a, ctl);
+                hipLaunchKernelGGL(k_peel_decrement, dim3(gs), dim3(256), 0, s, a, ctl);
+                hipLaunchKernelGGL(k_peel_ctl_round, dim3(1), dim3(1), 0, s, ctl, a.cnt, maxdeg);
+            }
+            KC_HIP(hipGetLastError());
+            KC_HIP(hipMemcpyAsync(&phase, &ctl->phase, sizeof phase, hipMemcpyDeviceToHost, s));
+            KC_HIP(hipStreamSynchronize(s));
+        }
+    }
     uint32_t h[4];
     KC_HIP(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
     KC_HIP(hipStreamSynchronize(s));
@@ -618,3 +776,130 @@ int kc_oriented_export(const kc_oriented *o, uint32_t *rowptr, uint32_t *col, ui
 void kc_oriented_free(kc_oriented *o) { delete o; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// binary oriented-CSR cache
+// ---------------------------------------------------------------------------
+namespace {
+
+const char kCacheMagic[8] = {'K', 'C', 'O', 'C', 'S', 'R', '1', '\0'};
+
+struct CacheHeader {
+    char magic[8];
+    uint32_t n, mode;
+    uint64_t m;
+    uint32_t max_out, max_core;
+    uint64_t checksum;
+};
+static_assert(sizeof(CacheHeader) == 40, "cache header is 40 bytes");
+
+uint64_t fnv1a_words(uint64_t h, const uint32_t *p, size_t count)
+{
+    for (size_t i = 0; i < count; ++i)
+        for (int b = 0; b < 4; ++b) {
+            h ^= (p[i] >> (8 * b)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    return h;
+}
+
+uint64_t cache_checksum(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &col,
+                        const std::vector<uint32_t> &key)
+{
+    uint64_t h = 14695981039346656037ull;
+    h = fnv1a_words(h, rp.data(), rp.size());
+    h = fnv1a_words(h, col.data(), col.size());
+    return fnv1a_words(h, key.data(), key.size());
+}
+
+}  // namespace
+
+extern "C" int kc_oriented_save(const kc_oriented *o, const char *path)
+{
+    if (!o || !path) return fail(KC_ERR_ARG, "kc_oriented_save: null argument");
+    std::vector<uint32_t> rp((size_t)o->n + 1), col(o->m), key(o->n);
+    int rc = kc_oriented_export(o, rp.data(), col.data(), key.data());
+    if (rc) return rc;
+    CacheHeader h;
+    std::memcpy(h.magic, kCacheMagic, 8);
+    h.n = o->n;
+    h.mode = (uint32_t)o->mode;
+    h.m = o->m;
+    h.max_out = o->max_out;
+    h.max_core = o->max_core;
+    h.checksum = cache_checksum(rp, col, key);
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(KC_ERR_IO, "cannot create '%s'", path);
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(rp.data(), 4, rp.size(), f) == rp.size() &&
+              std::fwrite(col.data(), 4, col.size(), f) == col.size() &&
+              std::fwrite(key.data(), 4, key.size(), f) == key.size();
+    if (std::fclose(f) != 0) ok = false;
+    return ok ? KC_OK : fail(KC_ERR_IO, "write to '%s' failed", path);
+}
+
+extern "C" int kc_oriented_load(const char *path, kc_oriented **out)
+{
+    if (!path || !out) return fail(KC_ERR_ARG, "kc_oriented_load: null argument");
+    *out = nullptr;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(KC_ERR_IO, "cannot open '%s'", path);
+    CacheHeader h;
+    std::vector<uint32_t> rp, col, key;
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, kCacheMagic, 8) == 0 &&
+              (h.mode == KC_ORIENT_DEGREE || h.mode == KC_ORIENT_DEGEN) && h.m < 0xFFFFFFFFull && h.n < 0xFFFFFFFFu;
+    if (ok) {
+        rp.resize((size_t)h.n + 1);
+        col.resize(h.m);
+        key.resize(h.n);
+        ok = std::fread(rp.data(), 4, rp.size(), f) == rp.size() && std::fread(col.data(), 4, col.size(), f) == col.size() &&
+             std::fread(key.data(), 4, key.size(), f) == key.size() && std::fgetc(f) == EOF;
+    }
+    std::fclose(f);
+    if (!ok) return fail(KC_ERR_FORMAT, "'%s' is not an oriented-CSR cache file", path);
+    if (cache_checksum(rp, col, key) != h.checksum) return fail(KC_ERR_FORMAT, "'%s': checksum mismatch", path);
+    // well formed: monotone rowptr ending at m, sorted rows of ids < n, and
+    // every arc going up in (key, id) order (so the graph is acyclic)
+    uint32_t max_out = 0;
+    if (rp[0] != 0 || rp[h.n] != h.m) return fail(KC_ERR_FORMAT, "'%s': bad row pointers", path);
+    for (uint32_t u = 0; u < h.n; ++u) {
+        if (rp[u + 1] < rp[u]) return fail(KC_ERR_FORMAT, "'%s': bad row pointers", path);
+        max_out = std::max(max_out, rp[u + 1] - rp[u]);
+        for (uint32_t i = rp[u]; i < rp[u + 1]; ++i) {
+            uint32_t v = col[i];
+            if (v >= h.n || (i > rp[u] && col[i - 1] >= v) || key[u] > key[v] || (key[u] == key[v] && u >= v))
+                return fail(KC_ERR_FORMAT, "'%s': row %u is not a sorted upward out-list", path, u);
+        }
+    }
+    if (max_out != h.max_out) return fail(KC_ERR_FORMAT, "'%s': max out-degree does not match the rows", path);
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    kc_oriented *o = new kc_oriented;
+    o->dev = ctx->dev;
+    o->stream = ctx->stream;
+    o->n = h.n;
+    o->m = h.m;
+    o->mode = (int)h.mode;
+    o->max_out = max_out;
+    o->max_core = h.max_core;
+    std::vector<uint32_t> src(h.m);
+    for (uint32_t u = 0; u < h.n; ++u)
+        for (uint32_t i = rp[u]; i < rp[u + 1]; ++i) src[i] = u;
+    if ((rc = o->rowptr.ensure(4 * ((size_t)h.n + 1))) || (rc = o->col.ensure(4 * (size_t)h.m + 4)) ||
+        (rc = o->src.ensure(4 * (size_t)h.m + 4)) || (rc = o->key.ensure(4 * (size_t)h.n + 4))) {
+        delete o;
+        return rc;
+    }
+    hipStream_t s = o->stream;
+    bool up = hipMemcpyAsync(o->rowptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+              (h.m == 0 || (hipMemcpyAsync(o->col.p, col.data(), 4 * col.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+                            hipMemcpyAsync(o->src.p, src.data(), 4 * src.size(), hipMemcpyHostToDevice, s) == hipSuccess)) &&
+              (h.n == 0 || hipMemcpyAsync(o->key.p, key.data(), 4 * key.size(), hipMemcpyHostToDevice, s) == hipSuccess) &&
+              hipStreamSynchronize(s) == hipSuccess;
+    if (!up) {
+        delete o;
+        return fail(KC_ERR_DEVICE, "oriented-CSR upload failed");
+    }
+    *out = o;
+    return KC_OK;
+}

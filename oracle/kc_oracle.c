@@ -20,6 +20,9 @@
  *                          kclique/kckernels.cuh:2263-2630), with the
  *                         task filters of kclique/kclique.cuh:1609-1614,
  *                         :2344-2355
+ *   - orc_count_local   : per-vertex counts of `-m lkc`
+ *                         (kclique/kclique_local.cuh:24-639 with the kernels
+ *                          of kclique/kckernels_local.cuh)
  *   - orc_binom         : exact binomial coefficients that replace the
  *                         double-built nCr table (kclique/kclique_local.cuh:345-361)
  *   - orc_brute_force   : all-k-subsets counter for tiny graphs (n <= 64);
@@ -515,6 +518,274 @@ int orc_count(u32 n, const u64 *rowptr, const u32 *col, u32 k, int algo, int gra
     *tasks_done = td;
     *alg_bytes = (u64)tb;
     free(part); free(pbytes); free(ptasks); free(povf); free(src_of);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Local (per-vertex) counts, `-m lkc` (kclique/kclique_local.cuh:24-639,    */
+/* kernels kclique/kckernels_local.cuh): local[v] = number of k-cliques that  */
+/* contain v.  The reference adds, per found clique or SCT leaf, to each     */
+/* member: orientation adds 1 per member of an enumerated clique; pivoting   */
+/* splits a leaf with h held and p pivot vertices into C(p, k-h) for each     */
+/* held vertex and C(p-1, k-h-1) for each pivot vertex.                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    ctx_t c;
+    u128 *loc;        /* per-thread local counts, n entries */
+    u32 roots[2];     /* the task's root vertices (held) */
+    u32 nroots;
+    u32 *H, nh;       /* held task-local indices on the current path */
+    u32 *P, np;       /* pivot task-local indices on the current path */
+} lctx_t;
+
+static u128 binom_or0(lctx_t *L, int64_t n, int64_t r)
+{
+    if (r < 0 || n < 0 || r > n) return 0;
+    u128 v = binom128((u32)n, (u32)r);
+    if (v > (u128)UINT64_MAX) L->c.ovf = 1;
+    return v;
+}
+
+static void add_held(lctx_t *L, u128 v)
+{
+    if (!v) return;
+    for (u32 i = 0; i < L->nroots; ++i) L->loc[L->roots[i]] += v;
+    for (u32 i = 0; i < L->nh; ++i) L->loc[L->c.list[L->H[i]]] += v;
+}
+
+/* Orientation DFS with the held path: `need` more vertices from cand. */
+static void dag_local(lctx_t *L, const u64 *cand, u32 need, u32 depth)
+{
+    u32 W = L->c.W;
+    if (need == 0) { add_held(L, 1); return; }
+    if (need == 1) {
+        u32 cnt = 0;
+        for (u32 wi = 0; wi < W; ++wi) {
+            u64 b = cand[wi];
+            while (b) {
+                u32 j = wi * 64 + (u32)__builtin_ctzll(b);
+                b &= b - 1;
+                L->loc[L->c.list[j]] += 1;
+                cnt++;
+            }
+        }
+        add_held(L, cnt);
+        return;
+    }
+    u64 *child = L->c.stack + (size_t)depth * 2 * W;
+    for (u32 wi = 0; wi < W; ++wi) {
+        u64 bits = cand[wi];
+        while (bits) {
+            u32 j = wi * 64 + (u32)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            const u64 *rj = L->c.rows + (size_t)j * W;
+            int cnt = 0;
+            for (u32 w = 0; w < W; ++w) { child[w] = cand[w] & rj[w]; cnt += popc64(child[w]); }
+            if ((u32)cnt < need - 1) continue;
+            L->H[L->nh++] = j;
+            dag_local(L, child, need - 1, depth + 1);
+            L->nh--;
+        }
+    }
+}
+
+/* SCT with the held and pivot sets of the path.  Without tails the leaves
+ * carry the reference's hold/pivot split.  With tails a node with t = k - h
+ * <= 3 closes its subtree: its cliques are H + S + Q with S a subset of the
+ * pivots P and Q a clique of G[C], |S| + |Q| = t, so
+ *   held vertex   sum_j C(p, t-j)   c_j(C)
+ *   pivot vertex  sum_j C(p-1, t-1-j) c_j(C)
+ *   x in C        sum_j C(p, t-j)   c_{j-1}(C & N(x))
+ * with c_0 = 1, c_1 = size, c_2 = edges, c_3 = triangles. */
+static void sct_local(lctx_t *L, const u64 *C, u32 depth)
+{
+    u32 W = L->c.W, k = L->c.k;
+    u32 held = L->nroots + L->nh, piv = L->np;
+    if (held > k) return;
+    u32 cs = (u32)popc_set(C, W);
+    int64_t t = (int64_t)k - held;
+    if (cs == 0) {
+        add_held(L, binom_or0(L, piv, t));
+        u128 pv = binom_or0(L, (int64_t)piv - 1, t - 1);
+        if (pv) for (u32 i = 0; i < L->np; ++i) L->loc[L->c.list[L->P[i]]] += pv;
+        return;
+    }
+    if (held + piv + cs < k) return;
+    if (L->c.tails && t <= 3) {
+        /* per-candidate degree and edge count inside C */
+        u128 c1 = cs, c2 = 0, c3 = 0;
+        u64 *nx = L->c.stack + (size_t)depth * 3 * W;
+        for (u32 wi = 0; wi < W; ++wi) {
+            u64 bi = C[wi];
+            while (bi) {
+                u32 x = wi * 64 + (u32)__builtin_ctzll(bi);
+                bi &= bi - 1;
+                const u64 *rx = L->c.rows + (size_t)x * W;
+                u32 dx = 0;
+                for (u32 w = 0; w < W; ++w) { nx[w] = C[w] & rx[w]; dx += (u32)popc64(nx[w]); }
+                u128 ex2 = 0;  /* twice the edges inside C & N(x) */
+                if (t == 3)
+                    for (u32 wj = 0; wj < W; ++wj) {
+                        u64 bj = nx[wj];
+                        while (bj) {
+                            u32 y = wj * 64 + (u32)__builtin_ctzll(bj);
+                            bj &= bj - 1;
+                            const u64 *ry = L->c.rows + (size_t)y * W;
+                            for (u32 w = 0; w < W; ++w) ex2 += (u128)popc64(nx[w] & ry[w]);
+                        }
+                    }
+                c2 += dx;
+                c3 += ex2 / 2;
+                u128 add = binom_or0(L, piv, t - 1) + binom_or0(L, piv, t - 2) * dx + binom_or0(L, piv, t - 3) * (ex2 / 2);
+                L->loc[L->c.list[x]] += add;
+            }
+        }
+        c2 /= 2;
+        c3 /= 3;
+        u128 cj[4] = {1, c1, c2, c3};
+        u128 tot = 0, ptot = 0;
+        for (int64_t j = 0; j <= t; ++j) {
+            tot += binom_or0(L, piv, t - j) * cj[j];
+            ptot += binom_or0(L, (int64_t)piv - 1, t - 1 - j) * cj[j];
+        }
+        add_held(L, tot);
+        if (ptot) for (u32 i = 0; i < L->np; ++i) L->loc[L->c.list[L->P[i]]] += ptot;
+        return;
+    }
+    int best = -1; u32 bj = 0;
+    for (u32 wi = 0; wi < W; ++wi) {
+        u64 bits = C[wi];
+        while (bits) {
+            u32 j = wi * 64 + (u32)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            const u64 *rj = L->c.rows + (size_t)j * W;
+            int cnt = 0;
+            for (u32 w = 0; w < W; ++w) cnt += popc64(C[w] & rj[w]);
+            if (cnt > best) { best = cnt; bj = j; }
+        }
+    }
+    if (held + piv + 1 + (u32)best < k) return;
+    u64 *Cc = L->c.stack + (size_t)depth * 3 * W;
+    u64 *B = Cc + W;
+    u64 *child = B + W;
+    const u64 *rp = L->c.rows + (size_t)bj * W;
+    for (u32 w = 0; w < W; ++w) { Cc[w] = C[w]; B[w] = C[w] & ~rp[w]; }
+    for (u32 wi = 0; wi < W; ++wi) {
+        while (B[wi]) {
+            u32 b = wi * 64 + (u32)__builtin_ctzll(B[wi]);
+            B[wi] &= B[wi] - 1;
+            const u64 *rb = L->c.rows + (size_t)b * W;
+            for (u32 w = 0; w < W; ++w) child[w] = Cc[w] & rb[w];
+            if (b == bj) { L->P[L->np++] = b; sct_local(L, child, depth + 1); L->np--; }
+            else { L->H[L->nh++] = b; sct_local(L, child, depth + 1); L->nh--; }
+            Cc[wi] &= ~(1ull << (b & 63));
+        }
+    }
+}
+
+/*
+ * orc_count_local: local[v] = number of k-cliques of the oriented graph
+ * (rowptr, col) that contain v, by the traversal `algo` (0 orientation,
+ * 1 pivoting, 2 pivoting with closed-form tails) over node or edge tasks.
+ * *overflow is set when an entry (or a binomial it needs) is >= 2^64.
+ * Sum over v of local[v] = k * (the k-clique count).
+ */
+int orc_count_local(u32 n, const u64 *rowptr, const u32 *col, u32 k, int algo, int gran, int nthreads,
+                    u64 *local, int *overflow)
+{
+    if (k < 1) return -1;
+    u64 m = rowptr[n];
+    *overflow = 0;
+    if (k <= 2) {
+        for (u32 v = 0; v < n; ++v) local[v] = (k == 1) ? 1 : rowptr[v + 1] - rowptr[v];
+        if (k == 2) for (u64 e = 0; e < m; ++e) local[col[e]]++;
+        return 0;
+    }
+    u32 maxd = 0;
+    for (u32 u = 0; u < n; ++u) {
+        u64 d = rowptr[u + 1] - rowptr[u];
+        if (d > maxd) maxd = (u32)d;
+    }
+    int nt = 1;
+#ifdef _OPENMP
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    (void)nthreads;
+#endif
+    u128 *all = (u128 *)calloc((size_t)nt * (n ? n : 1), sizeof(u128));
+    int *povf = (int *)calloc((size_t)nt, sizeof(int));
+    u64 ntasks = (gran == 0) ? n : m;
+    u32 *src_of = NULL;
+    if (gran == 1) {
+        src_of = (u32 *)malloc(sizeof(u32) * (m ? m : 1));
+        for (u32 u = 0; u < n; ++u)
+            for (u64 e = rowptr[u]; e < rowptr[u + 1]; ++e) src_of[e] = u;
+    }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        lctx_t L;
+        memset(&L, 0, sizeof L);
+        u32 Wmax = (maxd + 63) / 64;
+        if (Wmax == 0) Wmax = 1;
+        L.c.k = k;
+        L.c.rows = (u64 *)malloc(sizeof(u64) * ((size_t)maxd + 1) * Wmax);
+        L.c.stack = (u64 *)malloc(sizeof(u64) * ((size_t)maxd + k + 4) * 3 * Wmax);
+        L.c.list = (u32 *)malloc(sizeof(u32) * ((size_t)maxd + 1));
+        L.c.tails = (algo == 2);
+        L.H = (u32 *)malloc(sizeof(u32) * ((size_t)maxd + k + 4));
+        L.P = (u32 *)malloc(sizeof(u32) * ((size_t)maxd + k + 4));
+        L.loc = all + (size_t)tid * n;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (u64 t = 0; t < ntasks; ++t) {
+            u32 R;
+            if (gran == 0) {
+                u32 u = (u32)t;
+                u64 d = rowptr[u + 1] - rowptr[u];
+                if (d < k - 1) continue;
+                R = (u32)d;
+                memcpy(L.c.list, col + rowptr[u], sizeof(u32) * R);
+                L.roots[0] = u;
+                L.nroots = 1;
+            } else {
+                u32 u = src_of[t], v = col[t];
+                u64 du = rowptr[u + 1] - rowptr[u], dv = rowptr[v + 1] - rowptr[v];
+                R = intersect(col + rowptr[u], du, col + rowptr[v], dv, L.c.list);
+                if (R < k - 2) continue;
+                L.roots[0] = u;
+                L.roots[1] = v;
+                L.nroots = 2;
+            }
+            L.c.W = (R + 63) / 64;
+            if (L.c.W == 0) L.c.W = 1;
+            encode(&L.c, rowptr, col, R, algo != 0);
+            L.nh = L.np = 0;
+            u64 *C = L.c.stack + (size_t)(maxd + k + 3) * 3 * Wmax - Wmax;
+            memset(C, 0, sizeof(u64) * L.c.W);
+            for (u32 j = 0; j < R; ++j) C[j >> 6] |= 1ull << (j & 63);
+            if (algo == 0) dag_local(&L, C, k - L.nroots, 0);
+            else sct_local(&L, C, 0);
+        }
+        povf[tid] = L.c.ovf;
+        free(L.c.rows); free(L.c.stack); free(L.c.list); free(L.H); free(L.P);
+    }
+    int ov = 0;
+    for (int i = 0; i < nt; ++i) ov |= povf[i];
+    for (u32 v = 0; v < n; ++v) {
+        u128 s = 0;
+        for (int i = 0; i < nt; ++i) s += all[(size_t)i * n + v];
+        if (s > (u128)UINT64_MAX) ov = 1;
+        local[v] = (u64)s;
+    }
+    *overflow = ov;
+    free(all); free(povf); free(src_of);
     return 0;
 }
 

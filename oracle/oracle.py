@@ -10,6 +10,7 @@ C restatements (``kc_oracle.c``, loaded with ctypes):
   * :func:`peel_priority`   -- kcore/kcore.cuh:442-584
   * :func:`count`           -- kclique/kckernels_orientation.cuh, kckernels_pivoting.cuh,
                                kckernels.cuh:2263-2630
+  * :func:`count_local`     -- kclique/kclique_local.cuh:24-639 (``-m lkc``)
   * :func:`binom`, :func:`brute_force`
 """
 from __future__ import annotations
@@ -44,6 +45,9 @@ def _lib():
         lib.orc_count.argtypes = [ctypes.c_uint32, u64p, u32p, ctypes.c_uint32, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                   u64p, ip, u64p, u64p]
+        lib.orc_count_local.restype = ctypes.c_int
+        lib.orc_count_local.argtypes = [ctypes.c_uint32, u64p, u32p, ctypes.c_uint32, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, u64p, ip]
         lib.orc_brute_force.restype = ctypes.c_uint64
         lib.orc_brute_force.argtypes = [ctypes.c_uint32, u64p, ctypes.c_uint32, ip]
         lib.orc_threads.restype = ctypes.c_int
@@ -157,6 +161,20 @@ def count(orowptr, ocol, k, algo="pivot", gran="node", stride=1, offset=0, threa
     if rc != 0:
         raise ValueError("orc_count failed")
     return {"count": c.value, "overflow": bool(o.value), "tasks": t.value, "bytes": b.value}
+
+
+def count_local(orowptr, ocol, k, algo="pivot", gran="node", threads=0):
+    """Per-vertex k-clique counts (``-m lkc``): (uint64 array of n, overflow)."""
+    orowptr = np.ascontiguousarray(orowptr, dtype=np.uint64)
+    ocol = np.ascontiguousarray(ocol, dtype=np.uint32)
+    n = orowptr.size - 1
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    o = ctypes.c_int(0)
+    rc = _lib().orc_count_local(n, _p(orowptr, ctypes.c_uint64), _p(ocol, ctypes.c_uint32), int(k),
+                                ALGO[algo], GRAN[gran], int(threads), _p(out, ctypes.c_uint64), ctypes.byref(o))
+    if rc != 0:
+        raise ValueError("orc_count_local failed")
+    return out[:n], bool(o.value)
 
 
 def binom(n, r):

@@ -1,6 +1,6 @@
 """Writes tests/golden/cliques.json: small graphs with their k-clique counts.
 
-Counts come from networkx (``enumerate_all_cliques``), an implementation
+Counts (total, and per vertex for k = 3..6) come from networkx (``enumerate_all_cliques``), an implementation
 independent of both the oracle and the HIP kernels, run in the build
 container only; the JSON is what the tests read.  Closed-form families are
 added with their formulas so that a mistake in networkx use would also show.
@@ -26,6 +26,21 @@ def counts(G):
     return {k: c.get(k, 0) for k in range(1, KMAX + 1)}
 
 
+LOCAL_K = (3, 4, 5, 6)
+
+
+def local_counts(G):
+    """per-vertex k-clique counts for k in LOCAL_K (``-m lkc``)"""
+    loc = {k: [0] * G.number_of_nodes() for k in LOCAL_K}
+    for cl in nx.enumerate_all_cliques(G):
+        if len(cl) in loc:
+            for v in cl:
+                loc[len(cl)][v] += 1
+        if len(cl) > max(LOCAL_K):
+            break
+    return loc
+
+
 def entry(name, G, closed=None):
     G = nx.convert_node_labels_to_integers(G)
     e = {
@@ -34,6 +49,7 @@ def entry(name, G, closed=None):
         "edges": sorted([min(u, v), max(u, v)] for u, v in G.edges()),
         "counts": counts(G),
         "max_core": max(nx.core_number(G).values()) if G.number_of_edges() else 0,
+        "local": local_counts(G),
     }
     if closed is not None:
         assert all(e["counts"][k] == closed(k) for k in range(1, KMAX + 1)), name

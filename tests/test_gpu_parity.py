@@ -53,6 +53,22 @@ def test_orientation_matches_oracle(golden, mode):
         assert Og.info()["max_out_degree"] == int(np.diff(erp).max(initial=0))
 
 
+def test_stepped_peel_matches_oracle(golden, monkeypatch):
+    """The peel without a cooperative launch (taken when the device's
+    cooperative resource is held by another process) gives the same key."""
+    monkeypatch.setenv("KC_PEEL_MODE", "host")
+    graphs = [_graph(g) for g in golden]
+    graphs.append(K.Graph.generate(K.GEN_CHUNG_LU, seed=3, n=20000, m=150000, gamma=2.1, max_degree=2000))
+    for G in graphs:
+        rp, col = _host_csr(G)
+        pr, mc = O.peel_priority(rp, col)
+        Og = G.orient("degen")
+        orp, oc, key = Og.export()
+        assert (key == pr).all() and Og.info()["max_core"] == mc
+        erp, ec = O.orient(rp, col, pr)
+        assert (orp == erp).all() and (oc == ec).all()
+
+
 @pytest.mark.parametrize("mode", ["degree", "degen"])
 @pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("gran", GRANS)
@@ -236,8 +252,13 @@ def test_serving_results_equal_alone():
     try:
         gids = [srv.add_graph(G, "degen" if i % 2 else "degree") for i, G in enumerate(graphs)]
         ors = {gid: G.orient("degen" if i % 2 else "degree") for i, (gid, G) in enumerate(zip(gids, graphs))}
-        qs = [(gids[int(rng.integers(0, len(gids)))], int(rng.integers(1, 10)), ALGOS[int(rng.integers(0, 2))],
-               GRANS[int(rng.integers(0, 2))]) for _ in range(120)]
+        qs = []
+        for _ in range(120):
+            gi = int(rng.integers(0, len(gids)))
+            algo = ALGOS[int(rng.integers(0, 2))]
+            if gi == len(gids) - 1:
+                algo = "pivot"  # enumerating C(150, 9) cliques one by one takes hours
+            qs.append((gids[gi], int(rng.integers(1, 10)), algo, GRANS[int(rng.integers(0, 2))]))
         tickets = [None] * len(qs)
 
         def submit(lo, hi):  # several client threads, queries arrive while others run
@@ -271,3 +292,56 @@ def test_serving_results_equal_alone():
         assert ei.value.code == K.KC_ERR_NOTFOUND
     finally:
         srv.close()
+
+
+def test_cli_end_to_end(tmp_path, golden):
+    """kcgpu -g file -m kc -o ... -k ... -p ... -q ...: the reference's flags
+    and output lines, with the exact count."""
+    import os
+    import re
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kcgpu_amd", "bin", "kcgpu")
+    g = next(x for x in golden if x["name"] == "planted_200")
+    p = tmp_path / "g.mtx"
+    lines = ["%%MatrixMarket matrix coordinate pattern symmetric", f"{g['n']} {g['n']} {len(g['edges'])}"]
+    lines += [f"{b + 1} {a + 1}" for a, b in g["edges"]]  # lower triangle, 1-based
+    p.write_text("\n".join(lines) + "\n")
+    for orient, by, spec, k in (("degree", "node", "o8b", 5), ("degen", "edge", "p64b", 7), ("degen", "node", "p1n", 9)):
+        r = subprocess.run([cli, "-g", str(p), "-d", "0", "-m", "kc", "-o", orient, "-k", str(k), "-p", by,
+                            "-q", spec, "-w", "0"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        m = re.search(r"Counter = (\d+)", r.stdout)
+        assert m and int(m.group(1)) == g["counts"][k], r.stdout
+        assert re.search(r"MOHA %d k-clique" % g["counts"][k], r.stdout)
+        assert f"kcliques={g['counts'][k]} k={k} overflow=0" in r.stdout
+        if orient == "degen":
+            assert f"Max Core = {g['max_core']}" in r.stdout
+
+
+def test_oriented_cache_round_trip(tmp_path, golden):
+    """save -> load gives the same oriented CSR, key and counts; the CLI counts
+    a .kco file without orienting again."""
+    import os
+    import re
+    import subprocess
+    g = next(x for x in golden if x["name"] == "planted_200")
+    e = np.array(g["edges"], dtype=np.uint32).reshape(-1, 2)
+    G = K.Graph.from_edges(e[:, 0], e[:, 1], g["n"])
+    for mode in ("degree", "degen"):
+        Og = G.orient(mode)
+        p = tmp_path / f"g_{mode}.kco"
+        Og.save(p)
+        L = K.Oriented.load(p)
+        a, b = Og.export(), L.export()
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+        assert L.info()["max_out_degree"] == Og.info()["max_out_degree"]
+        assert L.info()["max_core"] == Og.info()["max_core"]
+        for k in (4, 7):
+            for gran in GRANS:
+                assert L.count(k, "pivot", gran, part_size=8).count == g["counts"][k]
+                assert L.count(k, "orient", gran, part_size=8).count == g["counts"][k]
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kcgpu_amd", "bin", "kcgpu")
+    r = subprocess.run([cli, "-g", str(tmp_path / "g_degen.kco"), "-m", "kc", "-k", "6", "-p", "edge", "-q", "p8b"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"kcliques={g['counts'][6]} k=6 overflow=0" in r.stdout
