@@ -234,6 +234,46 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def run_config(args):
+    """The other single-graph configurations, counted per (orientation,
+    granularity) with every result cross-checked:
+      c1: Erdos-Renyi G(n=10,000, m=100,000), seed 1, degree orientation,
+          node tasks, orientation DFS with P = 8 (-q o8b), k = 3 and 4;
+      c3: R-MAT scale 22, edge factor 16, (a,b,c) = (.57,.19,.19), seed 3,
+          {degree, degen} x {node, edge} pivoting, k = 3..11 (--ks)."""
+    import numpy as np
+    import kcgpu_amd as K
+    if args.workload == "c1":
+        G = K.Graph.generate(K.GEN_ER, seed=1, n=10000, m=100000)
+        combos, ks, algo, P = [("degree", "node")], [3, 4], "orient", 8
+    else:
+        G = K.Graph.generate(K.GEN_RMAT, seed=3, scale=22, edge_factor=16, a=0.57, b=0.19, c=0.19)
+        combos = [tuple(c.split(":")) for c in args.combos.split(",")]
+        ks, algo, P = [int(x) for x in args.c3_ks.split(",")], "pivot", 1
+    gi = G.info()
+    log("graph", gi)
+    ors, rows = {}, []
+    for o, g in combos:
+        if o not in ors:
+            ors[o] = G.orient(o)
+            log("oriented", o, ors[o].info())
+        for k in ks:
+            r = ors[o].count(k, algo, g, part_size=P, check=False)
+            ms = r.raw["count_ms"]
+            m_or = ors[o].info()["m"]
+            row = {"orient": o, "gran": g, "k": k, "count": r.count, "overflow": r.overflow, "device_ms": round(ms, 3),
+                   "kcliques_per_s": r.count / (ms / 1e3) if ms else None,
+                   "edges_per_s": m_or / (ms / 1e3) if ms else None}
+            rows.append(row)
+            log(json.dumps(row))
+    agree = all(len({(x["count"], x["overflow"]) for x in rows if x["k"] == k}) == 1 for k in ks)
+    print(json.dumps({"workload": args.workload, "graph": gi, "agree_across_modes": agree,
+                      "orient_ms": {o: round(v.info()["preprocess_ms"], 3) for o, v in ors.items()},
+                      "max_core": {o: v.info()["max_core"] for o, v in ors.items()}, "counts": rows}), flush=True)
+    if not agree:
+        sys.exit(1)
+
+
 def run_copy(args):
     """Counter calibration: device-to-device copies of a known size through the
     HIP runtime's copy kernel, so rocprofv3's FETCH_SIZE / WRITE_SIZE can be
@@ -265,11 +305,13 @@ def main():
     ap.add_argument("--ks", default="5,6,7,8")
     ap.add_argument("--part-size", type=int, default=0, help="0 = autotune")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=20000)
+    ap.add_argument("--cpu-stride", type=int, default=2000)
     ap.add_argument("--copy-mib", type=int, default=1024)
-    ap.add_argument("--workload", choices=["c2", "batch", "serve", "copy"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batch", "serve", "copy", "c1", "c3"], default="c2",
                     help="c2: the headline k=5..8 count (default); batch: the 4096-graph batched mix; "
-                         "serve: the serving loop")
+                         "serve: the serving loop; c1 / c3: the ER and R-MAT configurations")
+    ap.add_argument("--combos", default="degen:edge,degen:node,degree:edge,degree:node")
+    ap.add_argument("--c3-ks", default="3,4,5,6,7,8,9,10,11")
     ap.add_argument("--graphs", type=int, default=4096)
     ap.add_argument("--resident", type=int, default=512)
     ap.add_argument("--uploads", type=int, default=64)
@@ -290,6 +332,8 @@ def main():
         return run_serve(args)
     if args.workload == "copy":
         return run_copy(args)
+    if args.workload in ("c1", "c3"):
+        return run_config(args)
 
     import numpy as np
     import kcgpu_amd as K

@@ -221,6 +221,18 @@ typedef struct {
 int kc_count(const kc_oriented *o, const kc_query *q, kc_result *result);
 
 /*
+ * Per-vertex counts (-m lkc): local[v] = number of k-cliques that contain v,
+ * for every v < n (local has n entries).  result->count is the total count
+ * (the sum of local[] is k times it); no entry can wrap unless the total
+ * does, so result->overflow covers both.  Replaces
+ * SingleGPU_Kclique_Local::findKclqueIncremental_{node,edge}_{binary,pivot}_async_local
+ * (kclique/kclique_local.cuh:24-639, dispatched by src/main.cu:589-748);
+ * unlike the reference it also takes 'n' specs (same kernels) and never
+ * reads past local[n-1].
+ */
+int kc_count_local(const kc_oriented *o, const kc_query *q, uint64_t *local, kc_result *result);
+
+/*
  * Parse the reference's -q grammar: [o|p]<P>[b|n|h] (include/Config.h:235-271).
  * Unlike the reference, a spec of two characters ("p8") is parsed too (the
  * reference silently ignores specs shorter than 3 characters) and an invalid

@@ -131,6 +131,8 @@ SIGNATURES = {
     "kc_oriented_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "kc_convert": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
     "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
+    "kc_count_local": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(Result)]),
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
     "kc_select_device": (ctypes.c_int, [ctypes.c_int]),
@@ -331,6 +333,22 @@ class Oriented:
         if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
             _check(rc)
         return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+def _count_local(self, k, algo="pivot", gran="node", part_size=8, binary=True, auto_tune=False, check=True):
+    """-m lkc: (per-vertex uint64 counts, CountResult of the total)."""
+    q = _query(k, algo, gran, part_size, binary, auto_tune)
+    n = self.info()["n"]
+    loc = np.zeros(max(n, 1), dtype=np.uint64)
+    r = Result()
+    rc = lib().kc_count_local(self._h, ctypes.byref(q), loc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                              ctypes.byref(r))
+    if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
+        _check(rc)
+    return loc[:n], CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+Oriented.count_local = _count_local
 
 
 def _query(k, algo="pivot", gran="node", part_size=1, binary=True, auto_tune=False):

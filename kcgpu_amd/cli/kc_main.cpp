@@ -23,6 +23,7 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <vector>
 
 #include "../../include/kc.h"
 
@@ -54,7 +55,7 @@ void usage()
     std::fprintf(stderr,
                  "\nUsage:  kcgpu [options]\n"
                  "    -g <graph file>   .mtx, .tsv, .txt or .bel\n"
-                 "    -m <task>         kc (k-clique count, default) or kcore (max core);\n"
+                 "    -m <task>         kc (k-clique count, default), lkc (per-vertex counts) or kcore;\n"
                  "                      mtx-bel, tsv-bel, tsv-mtx, bel-mtx, txt-bel convert -g into -r;\n"
                  "                      cache writes the oriented CSR of -g (-o) into -r (.kco)\n"
                  "    -r <file>         output of a conversion or of -m cache\n"
@@ -117,8 +118,8 @@ int main(int argc, char **argv)
             stamp("Converted %s -> %s (%s) in %f s", graph, dst, task.c_str(), now() - tc0);
             return 0;
         }
-    if (task != "kc" && task != "kcore" && task != "cache") {
-        std::fprintf(stderr, "error: -m %s is not part of this build (kc, kcore, cache, mtx-bel, tsv-bel, "
+    if (task != "kc" && task != "lkc" && task != "kcore" && task != "cache") {
+        std::fprintf(stderr, "error: -m %s is not part of this build (kc, lkc, kcore, cache, mtx-bel, tsv-bel, "
                              "tsv-mtx, bel-mtx, txt-bel)\n", task.c_str());
         return 1;
     }
@@ -185,6 +186,22 @@ int main(int argc, char **argv)
     }
 
     kc_result r;
+    if (task == "lkc") {
+        // src/main.cu:589-748: count, then "count time" and SingleGPU_Kclique_Local::show
+        // (kclique/kclique_local.cuh:621-629), which prints the first 50 vertices
+        std::vector<uint64_t> local(oi.n ? oi.n : 1);
+        double tl = now();
+        rc = kc_count_local(o, &q, local.data(), &r);
+        double tc = now() - tl;
+        if (rc && rc != KC_ERR_OVERFLOW) return die(rc, "count");
+        stamp("count time %f s (%f teps)", tc, tc > 0 ? (double)oi.m / tc : 0.0);
+        std::printf("Local clique counter for the first %u Nodes:\n", oi.n);
+        for (uint32_t i = 0; i < oi.n && i < 50; ++i) std::printf("%u\t%" PRIu64 "\n", i, local[i]);
+        std::printf("kcliques=%" PRIu64 " k=%d overflow=%u\n", r.count, k, r.overflow);
+        kc_oriented_free(o);
+        if (g) kc_graph_free(g);
+        return r.overflow ? KC_ERR_OVERFLOW : 0;
+    }
     double t2 = now();
     rc = kc_count(o, &q, &r);
     double tc = now() - t2;

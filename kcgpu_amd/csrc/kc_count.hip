@@ -74,15 +74,19 @@ struct CountArgs {
     uint32_t *qovf;         // per query
     uint64_t *gscr;
     uint64_t gscr_words;
+    uint64_t *local;        // per-vertex counts of the (single) query, or null
 };
 
 // ---- workspace layout (u64 units) -----------------------------------------
+// Per-vertex counting (LOCAL) adds, for pivoting, the held and pivot sets of
+// the current node and of every stack level, and for both traversals a u64
+// accumulator per task vertex (lacc) and the DFS path of each partition.
 struct Layout {
-    uint64_t misc, list, offs, starts, rows, stk, x, scal, total;
+    uint64_t misc, list, offs, starts, rows, stk, x, scal, lacc, path, total;
 };
 
 __host__ __device__ inline Layout make_layout(uint32_t rmax, uint32_t wmax, uint32_t levels, uint32_t nparts,
-                                              bool pivot)
+                                              bool pivot, bool local = false)
 {
     Layout L;
     L.misc = 0;
@@ -91,13 +95,18 @@ __host__ __device__ inline Layout make_layout(uint32_t rmax, uint32_t wmax, uint
     L.starts = L.offs + (rmax + 2) / 2;
     L.rows = L.starts + (rmax + 1) / 2;
     L.stk = L.rows + (uint64_t)rmax * wmax;
+    const uint64_t sets = local ? 4 : 2;  // per level: candidates, branches (+ held, pivots)
     if (pivot) {
-        L.x = L.stk + (uint64_t)levels * 2 * wmax;
-        L.scal = L.x + wmax;
-        L.total = L.scal + (uint64_t)levels * 2;
+        L.x = L.stk + (uint64_t)levels * sets * wmax;
+        L.scal = L.x + (local ? 3 : 1) * (uint64_t)wmax;  // X (+ current held, pivots)
+        L.lacc = L.scal + (uint64_t)levels * 2;
+        L.path = L.lacc + (local ? rmax : 0);
+        L.total = L.path;
     } else {
         L.x = L.scal = L.stk + (uint64_t)nparts * levels * 2 * wmax;
-        L.total = L.x;
+        L.lacc = L.x;
+        L.path = L.lacc + (local ? rmax : 0);
+        L.total = L.path + (local ? ((uint64_t)nparts * levels + 1) / 2 : 0);
     }
     return L;
 }
@@ -243,13 +252,27 @@ __device__ __attribute__((always_inline)) void build_rows(const Cur &g, uint32_t
 // take first-level rows from a per-task LDS counter (*next_row) and run an
 // independent DFS each; a lane only ever touches its own words (w = lip,
 // lip + P, ...), so the DFS needs no barrier.  Returns the lane's share.
+// LOCAL also adds, per found clique, 1 to each member in lacc (the task
+// vertices; the root is credited with the task's count by the caller):
+// partitions share lacc, so those adds are LDS atomics.
+template <bool LOCAL>
 __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint64_t *stk, uint32_t *next_row,
-                                 uint32_t R, uint32_t W, uint32_t r, uint64_t &nodes, uint32_t &ovf)
+                                 uint32_t R, uint32_t W, uint32_t r, uint64_t &nodes, uint32_t &ovf,
+                                 uint64_t *lacc, uint32_t *path)
 {
     const int lane = lane_id();
     uint64_t cnt = 0;
     if (r == 2) {
-        for (uint32_t i = lane; i < R * W; i += 64) cnt += (uint64_t)__popcll(rows[i]);
+        for (uint32_t i = lane; i < R * W; i += 64) {
+            const uint64_t x = rows[i];
+            cnt += (uint64_t)__popcll(x);
+            if (LOCAL && x) {
+                const uint32_t row = i / W, wbase = (i % W) * 64;
+                atomicAdd((unsigned long long *)&lacc[row], (unsigned long long)__popcll(x));
+                for (uint64_t b = x; b; b &= b - 1)
+                    atomicAdd((unsigned long long *)&lacc[wbase + (uint32_t)__builtin_ctzll(b)], 1ull);
+            }
+        }
         return cnt;
     }
     const uint32_t P = a.psize, NP = a.nparts;
@@ -257,6 +280,7 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
     if (part >= NP) return 0;
     const uint32_t wmax = a.wmax;
     uint64_t *base = stk + (uint64_t)part * a.levels * 2 * wmax;
+    uint32_t *mypath = LOCAL ? path + (uint64_t)part * a.levels : nullptr;
     const uint32_t rem0 = r - 1;  // vertices still to pick below a first-level row
     uint64_t my_nodes = 0;
     for (;;) {
@@ -293,6 +317,7 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
             }
             const uint32_t bw = b >> 6;
             if (bw % P == lip) I[bw] &= ~(1ull << (b & 63));
+            if (LOCAL && lip == 0) mypath[t] = b;
             uint64_t *Cn = C + 2 * wmax;
             const uint64_t *rb = rows + (uint64_t)b * W;
             uint32_t c = 0;
@@ -307,6 +332,17 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
             const uint32_t remc = rem0 - 1 - (uint32_t)t;  // picks left below b
             if (remc == 1) {
                 if (lip == 0) add_checked(cnt, c, ovf);
+                if (LOCAL && c) {
+                    // clique = root + j + path[0..t] + one vertex of Cn
+                    for (uint32_t w = lip; w < W; w += P)
+                        for (uint64_t x = Cn[w]; x; x &= x - 1)
+                            atomicAdd((unsigned long long *)&lacc[w * 64 + (uint32_t)__builtin_ctzll(x)], 1ull);
+                    if (lip == 0) {
+                        atomicAdd((unsigned long long *)&lacc[j], (unsigned long long)c);
+                        for (int l = 0; l <= t; ++l)
+                            atomicAdd((unsigned long long *)&lacc[mypath[l]], (unsigned long long)c);
+                    }
+                }
             } else if (c >= remc) {
                 ++t;
             }
@@ -332,25 +368,56 @@ struct Scal {
 // closes h = k-3 with c_3 = triangles of G[X].  A node is also dropped when
 // h + p + max_j|X & row_j| + 1 < k (the largest clique inside X has at most
 // maxIntersection + 1 vertices, cf. kclique/kckernels.cuh:2573-2584).
+//
+// LOCAL also credits every task vertex in lacc with the k-cliques of the
+// subtree that contain it (the root with the task's count, by the caller).
+// A subtree (H, P, X) with t = k - h holds the cliques H + S + Q, S a subset
+// of P and Q a clique of G[X] with |S| + |Q| = t, so
+//     held vertex  sum_j C(p, t-j) c_j(X)      pivot vertex  sum_j C(p-1, t-1-j) c_j(X)
+//     x in X       sum_j C(p, t-j) c_{j-1}(X & row_x)
+// which at a leaf (X empty) is the reference's hold/pivot split
+// (kclique/kclique_local.cuh, kckernels_local.cuh).  The sets H and P of the
+// node live beside X, and each stack level keeps its own copies.  Vertex sets
+// H, P and X are disjoint and lane w owns the 64 vertices of word w, so the
+// adds need no atomics; no add can wrap unless the task count does.
+__device__ inline uint64_t binom_at(const CountArgs &a, int64_t n, int64_t r)
+{
+    return (r < 0 || n < 0 || r > n) ? 0ull : a.binom[(uint64_t)n * a.bstride + (uint64_t)r];
+}
+
+template <bool LOCAL>
 __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
                                 Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
-                                uint32_t &ovf)
+                                uint32_t &ovf, uint64_t *lacc)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
     const uint32_t wmax = a.wmax;
     const uint32_t P = a.psize, NPp = 64 / P, part = lane / P, lip = lane % P;
+    const uint32_t lvl = LOCAL ? 4 * wmax : 2 * wmax;  // stack words per level
+    uint64_t *Hc = X + wmax, *Pc = X + 2 * wmax;     // LOCAL: held / pivot sets of the node
     uint64_t acc = 0;
     // root candidate set = all R list vertices
     for (uint32_t w = lane; w < W; w += 64) {
         uint32_t lo = w * 64;
         X[w] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
+        if (LOCAL) Hc[w] = Pc[w] = 0;
     }
     __syncthreads();
     int t = -1;
     bool pending = true;
     uint32_t h = h0, p = 0;
     uint64_t my_nodes = 0;
+    // LOCAL: add vh to each held and vp to each pivot vertex of the node
+    auto credit_hp = [&](uint64_t vh, uint64_t vp) {
+        if (!LOCAL || (!vh && !vp)) return;
+        for (uint32_t w = lane; w < W; w += 64) {
+            if (vh)
+                for (uint64_t x = Hc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vh;
+            if (vp)
+                for (uint64_t x = Pc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vp;
+        }
+    };
     for (;;) {
         if (pending) {
             pending = false;
@@ -368,11 +435,17 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                     uint64_t idx = (uint64_t)p * a.bstride + (k - h);
                     if (a.binom_ovf[idx]) ovf = 1;
                     add_checked(acc, a.binom[idx], ovf);
+                    if (LOCAL) credit_hp(a.binom[idx], binom_at(a, (int64_t)p - 1, (int64_t)(k - h) - 1));
                 }
             } else if (h >= k) {
                 add_checked(acc, 1, ovf);
+                if (LOCAL) credit_hp(1, 0);
             } else if (h + 1 == k) {
                 add_checked(acc, (uint64_t)p + cx, ovf);
+                if (LOCAL) {
+                    credit_hp((uint64_t)p + cx, 1);
+                    for (uint32_t c = lane; c < cx; c += 64) lacc[cand[c]] += 1;
+                }
             } else if (h + p + cx >= k) {
                 // scores |X & row_j| for j in X: P lanes per candidate
                 uint32_t best_s = 0, best_j = INF32;
@@ -388,6 +461,8 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                         best_s = sc;
                         best_j = j;
                     }
+                    // h = k-2: x in X lies in C(p,1) + |X & row_x| of the subtree's cliques
+                    if (LOCAL && h + 2 == k && lip == 0) lacc[j] += (uint64_t)p + sc;
                 }
                 uint64_t key = (lip == 0 && best_j != INF32) ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
                 key = wave_max64(key);
@@ -395,18 +470,24 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                 const uint64_t tot_s = wave_sum64_checked(lip == 0 ? sum_s : 0ull, dummy);
                 const uint32_t max_s = (uint32_t)(key >> 32);
                 if (h + 2 == k) {
+                    const uint64_t before = acc;
                     add_checked(acc, (uint64_t)p * (p - (p ? 1 : 0)) / 2, ovf);
                     add_checked(acc, (uint64_t)p * cx, ovf);
                     add_checked(acc, tot_s / 2, ovf);
+                    if (LOCAL) credit_hp(acc - before, (p ? (uint64_t)p - 1 : 0) + cx);
                 } else if (h + p + max_s + 1 >= k) {
                     const uint32_t pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
                     ++t;
-                    uint64_t *Cc = stk + (uint64_t)t * 2 * wmax, *B = Cc + wmax;
+                    uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
                     const uint64_t *rp = rows + (uint64_t)pv * W;
                     for (uint32_t w = lane; w < W; w += 64) {
                         uint64_t x = X[w];
                         Cc[w] = x;
                         B[w] = x & ~rp[w];
+                        if (LOCAL) {
+                            B[2 * wmax + w] = Hc[w];
+                            B[3 * wmax + w] = Pc[w];
+                        }
                     }
                     if (lane == 0) scal[t] = Scal{h, p, pv, 0};
                     __syncthreads();
@@ -415,7 +496,7 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
         }
         if (t < 0) break;
         // next branch of level t: non-pivot vertices first, the pivot last
-        uint64_t *Cc = stk + (uint64_t)t * 2 * wmax, *B = Cc + wmax;
+        uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
         const Scal sc = scal[t];
         uint32_t b = INF32;
         for (uint32_t w = lane; w < W; w += 64) {
@@ -433,6 +514,11 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
         for (uint32_t w = lane; w < W; w += 64) {
             uint64_t c = Cc[w];
             X[w] = c & rb[w];
+            if (LOCAL) {
+                const uint64_t bit = (w == (b >> 6)) ? (1ull << (b & 63)) : 0ull;
+                Hc[w] = B[2 * wmax + w] | (b != sc.pivot ? bit : 0ull);
+                Pc[w] = B[3 * wmax + w] | (b == sc.pivot ? bit : 0ull);
+            }
             if (w == (b >> 6)) {
                 Cc[w] = c & ~(1ull << (b & 63));
                 B[w] &= ~(1ull << (b & 63));
@@ -481,12 +567,154 @@ __device__ inline uint64_t lds_row_word(const uint64_t *rows, uint32_t W, uint32
     return ((uint64_t)hi << 32) | lo;
 }
 
+// SCT of a subproblem of at most 64 vertices held one row per lane: lane i
+// keeps row i (bit j <=> vertex j of the subproblem is adjacent), X0 is the
+// candidate set, h0/p0 the held and pivot counts above it.  Every set is one
+// uniform 64-bit word, a score is one AND + popcount, a branch row is one
+// v_readlane pair, and the stack lives across the lanes (lane t = level t).
+// Nodes with h >= k-3 close in form as in pivot_reg.
+__device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t X0, uint32_t h0, uint32_t p0,
+                          uint64_t &nodes, uint32_t &ovf)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+    uint64_t X = X0, stkC = 0, stkB = 0;
+    uint32_t stkM = 0;
+    int t = -1;
+    bool pending = true;
+    uint32_t h = h0, p = p0;
+    uint64_t acc = 0, my_nodes = 0;
+    for (;;) {
+        if (pending) {
+            pending = false;
+            ++my_nodes;
+            const uint32_t cx = (uint32_t)__popcll(X);
+            if (cx == 0) {
+                if (k >= h && k - h <= p) {
+                    const uint64_t idx = (uint64_t)p * a.bstride + (k - h);
+                    if (a.binom_ovf[idx]) ovf = 1;
+                    add_checked(acc, a.binom[idx], ovf);
+                }
+            } else if (h >= k) {
+                add_checked(acc, 1, ovf);
+            } else if (h + 1 == k) {
+                add_checked(acc, (uint64_t)p + cx, ovf);
+            } else if (h + p + cx >= k) {
+                const bool mem = (X >> lane) & 1ull;
+                const uint32_t v = mem ? (uint32_t)__popcll(X & row) : 0u;
+                if (h + 3 >= k) {
+                    const uint32_t e2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(v));
+                    const uint64_t pp = p;
+                    if (h + 2 == k) {
+                        add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2, ovf);
+                        add_checked(acc, pp * cx, ovf);
+                        add_checked(acc, e2 / 2, ovf);
+                    } else {
+                        // triangles i < j < l of G[X], counted at i
+                        uint32_t tri = 0;
+                        const uint64_t mine = X & row & lane_above;
+                        for (uint64_t bits = X; bits; bits &= bits - 1) {
+                            const uint32_t il = (uint32_t)__builtin_ctzll(bits);
+                            const uint64_t above = il == 63 ? 0ull : (~0ull << (il + 1));
+                            const uint64_t Y = X & rdlane64(row, il) & above;
+                            if ((Y >> lane) & 1ull) tri += (uint32_t)__popcll(Y & mine);
+                        }
+                        tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
+                        add_checked(acc, pp * (pp - (pp ? 1 : 0)) * (pp >= 2 ? pp - 2 : 0) / 6, ovf);
+                        add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2 * cx, ovf);
+                        add_checked(acc, pp * (e2 / 2), ovf);
+                        add_checked(acc, tri, ovf);
+                    }
+                } else {
+                    // (score + 1, 63 - index): the max names the smallest index of the largest score
+                    uint32_t key = mem ? (((v + 1) << 8) | (63u - lane)) : 0u;
+                    key = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32(key));
+                    if ((key >> 8) + h + p >= k) {  // (key >> 8) = max score + 1 bounds the clique inside X
+                        const uint32_t pv = 63u - (key & 0xFFu);
+                        ++t;
+                        const uint64_t bw = X & ~rdlane64(row, pv);
+                        stkC = (lane == (uint32_t)t) ? X : stkC;
+                        stkB = (lane == (uint32_t)t) ? bw : stkB;
+                        const uint32_t M = h | (p << 8) | (pv << 20);
+                        stkM = (lane == (uint32_t)t) ? M : stkM;
+                    }
+                }
+            }
+        }
+        if (t < 0) break;
+        // next branch of level t: non-pivot vertices first, the pivot last
+        uint64_t Cc = rdlane64(stkC, (uint32_t)t), B = rdlane64(stkB, (uint32_t)t);
+        const uint32_t M = (uint32_t)__builtin_amdgcn_readlane((int)stkM, t);
+        const uint32_t hh = M & 0xFF, pp = (M >> 8) & 0xFFF, pv = M >> 20;
+        const uint64_t nb = B & ~(1ull << pv);
+        const bool last = nb == 0;
+        const uint32_t b = last ? pv : (uint32_t)__builtin_ctzll(nb);
+        X = Cc & rdlane64(row, b);
+        h = hh + (last ? 0u : 1u);
+        p = pp + (last ? 1u : 0u);
+        if (last) {
+            --t;  // the pivot branch replaces its level
+        } else {
+            Cc &= ~(1ull << b);
+            B &= ~(1ull << b);
+            stkC = (lane == (uint32_t)t) ? Cc : stkC;
+            stkB = (lane == (uint32_t)t) ? B : stkB;
+        }
+        if (h <= k) pending = true;
+    }
+    if (lane == 0) nodes += my_nodes;
+    return acc;
+}
+
+// Compaction: once a node's candidate set X has at most 64 vertices, its
+// subtree is re-encoded as a 64-vertex subproblem (vertex j = the j-th member
+// of X) and finished by sct64.  Lane i loads the full row of the i-th member
+// from the LDS copy; new row j is the ballot over lanes of bit x_j of those
+// rows (the bitmap is symmetric).  Cost ~ 6 instructions per member.
+template <int S>
+__device__ uint64_t compact_and_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t W,
+                                      const uint64_t (&X)[S], uint32_t *idx, uint32_t h, uint32_t p,
+                                      uint64_t &nodes, uint32_t &ovf)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t lt = lanemask_lt();
+    uint32_t before = 0, cx = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if ((X[s] >> lane) & 1ull) idx[before + (uint32_t)__popcll(X[s] & lt)] = 64u * s + lane;
+        before += (uint32_t)__popcll(X[s]);
+    }
+    cx = before;
+    __syncthreads();
+    uint64_t o[S];
+    const uint32_t mine = lane < cx ? idx[lane] : 0u;
+#pragma unroll
+    for (int w = 0; w < S; ++w) o[w] = (lane < cx && (uint32_t)w < W) ? rows[(uint64_t)mine * W + w] : 0ull;
+    __syncthreads();  // idx is reused by the next compaction
+    uint64_t row1 = 0;
+    uint32_t j = 0;
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        for (uint64_t bits = X[w]; bits; bits &= bits - 1, ++j) {
+            const uint32_t bj = (uint32_t)__builtin_ctzll(bits);
+            const uint64_t col = __ballot((o[w] >> bj) & 1ull);
+            row1 = (lane == j) ? col : row1;
+        }
+    }
+    const uint64_t X1 = cx >= 64 ? ~0ull : ((1ull << cx) - 1);
+    return sct64(a, k, row1, X1, h, p, nodes, ovf);
+}
+
 template <int S>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
-                              uint64_t &nodes, uint32_t &ovf)
+                              uint32_t *idx, uint64_t &nodes, uint32_t &ovf)
 {
     const uint32_t lane = (uint32_t)lane_id();
     const uint32_t W = (R + 63) / 64;
+    if (S == 1) {
+        const uint64_t r1 = lane < R ? rows[lane] : 0ull;
+        return sct64(a, k, r1, R >= 64 ? ~0ull : ((1ull << R) - 1), h0, 0, nodes, ovf);
+    }
     uint64_t row[S][S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -509,11 +737,13 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
     for (;;) {
         if (pending) {
             pending = false;
-            ++my_nodes;
             uint32_t cx = 0;
 #pragma unroll
             for (int w = 0; w < S; ++w) cx += (uint32_t)__popcll(X[w]);
-            if (cx == 0) {
+            if (cx <= 64 && cx > 0 && h + 3 <= k && h + p + cx >= k) {
+                // a subtree of at most 64 candidates: one word per row from here on
+                add_checked(acc, compact_and_count<S>(a, k, rows, W, X, idx, h, p, nodes, ovf), ovf);
+            } else if ((++my_nodes, cx == 0)) {
                 if (k >= h && k - h <= p) {
                     const uint64_t idx = (uint64_t)p * a.bstride + (k - h);
                     if (a.binom_ovf[idx]) ovf = 1;
@@ -763,7 +993,7 @@ __global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
             const uint32_t R = stage_task(g, task, list, w.bytes);
             __syncthreads();
             build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
-            add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, w.nodes, w.run_ovf), w.run_ovf);
+            add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf), w.run_ovf);
             __syncthreads();
             const uint64_t dt = wall_clock64() - t0;
             w.tmax = dt > w.tmax ? dt : w.tmax;
@@ -774,13 +1004,33 @@ __global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
 }
 
 // ---- the generic kernel: LDS (or HBM) bitmap, orientation or pivoting ------
-template <bool PIVOT, bool GLOBAL>
+// LOCAL: per-vertex counts of one query into a.local (the -m lkc path): each
+// task zeroes its vertex accumulators, the traversal fills them, and at the
+// end one HBM atomic per credited task vertex plus one per root vertex (the
+// root lies in every clique of its task) publishes them.
+template <bool LOCAL>
+__device__ __attribute__((always_inline)) void publish_local(const CountArgs &a, const Cur &g, uint32_t task,
+                                                             const uint32_t *list, const uint64_t *lacc, uint32_t R,
+                                                             uint64_t task_count)
+{
+    if (!LOCAL) return;
+    const int lane = lane_id();
+    for (uint32_t i = lane; i < R; i += 64)
+        if (lacc[i]) atomicAdd((unsigned long long *)&a.local[list[i]], (unsigned long long)lacc[i]);
+    if (lane == 0 && task_count) {
+        const uint32_t u = g.gran ? g.src[task] : task;
+        atomicAdd((unsigned long long *)&a.local[u], (unsigned long long)task_count);
+        if (g.gran) atomicAdd((unsigned long long *)&a.local[g.col[task]], (unsigned long long)task_count);
+    }
+}
+
+template <bool PIVOT, bool GLOBAL, bool LOCAL>
 __global__ __launch_bounds__(64) void k_count(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
     const int lane = lane_id();
     uint64_t *ws = GLOBAL ? a.gscr + (uint64_t)blockIdx.x * a.gscr_words : lds;
-    const Layout L = make_layout(a.rmax, a.wmax, a.levels, a.nparts, PIVOT);
+    const Layout L = make_layout(a.rmax, a.wmax, a.levels, a.nparts, PIVOT, LOCAL);
     uint32_t *misc = reinterpret_cast<uint32_t *>(ws + L.misc);
     uint32_t *list = reinterpret_cast<uint32_t *>(ws + L.list);
     uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);
@@ -789,6 +1039,8 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
     uint64_t *stk = ws + L.stk;
     uint64_t *X = ws + L.x;
     Scal *scal = reinterpret_cast<Scal *>(ws + L.scal);
+    uint64_t *lacc = ws + L.lacc;
+    uint32_t *path = reinterpret_cast<uint32_t *>(ws + L.path);
     WaveAcc w;
     for (;;) {
         uint32_t start = 0;
@@ -803,28 +1055,41 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
             const uint32_t k = g.k;
             ++w.ntask;
             const uint32_t R = stage_task(g, task, list, w.bytes);
+            if (LOCAL)
+                for (uint32_t i = lane; i < R; i += 64) lacc[i] = 0;
             if (g.gran && !PIVOT && k == 3) {  // edge orientation at k=3 adds |T|
                 add_checked(w.run, R, w.run_ovf);
+                if (LOCAL) {
+                    for (uint32_t i = lane; i < R; i += 64) lacc[i] = 1;
+                    __syncthreads();
+                    publish_local<LOCAL>(a, g, task, list, lacc, R, R);
+                    __syncthreads();
+                }
                 continue;
             }
             if (lane == 0) misc[0] = 0;
             __syncthreads();
             const uint32_t W = (R + 63) / 64;
+            uint64_t task_count;
             if (!PIVOT) {
                 // (k-1)-cliques of the DAG on N+(u) / (k-2)-cliques of the DAG on T
                 build_rows<false>(g, list, offs, starts, rows, R, W, w.bytes);
                 uint32_t povf = 0;
-                uint64_t part = orient_count(a, rows, stk, misc, R, W, g.gran ? k - 2 : k - 1, w.nodes, povf);
-                uint64_t sum = wave_sum64_checked(part, povf);
+                uint64_t part = orient_count<LOCAL>(a, rows, stk, misc, R, W, g.gran ? k - 2 : k - 1, w.nodes, povf,
+                                                    lacc, path);
+                task_count = wave_sum64_checked(part, povf);
                 w.run_ovf |= (wave_min32(povf ? 0u : 1u) == 0) ? 1u : 0u;
-                add_checked(w.run, sum, w.run_ovf);
+                add_checked(w.run, task_count, w.run_ovf);
             } else {
                 // SCT rooted at the node (held 1) or the edge (held 2)
                 build_rows<true>(g, list, offs, starts, rows, R, W, w.bytes);
-                uint64_t sum = pivot_count(a, k, rows, stk, X, scal, offs, R, W, g.gran ? 2u : 1u, w.nodes, w.run_ovf);
-                add_checked(w.run, sum, w.run_ovf);
+                task_count = pivot_count<LOCAL>(a, k, rows, stk, X, scal, offs, R, W, g.gran ? 2u : 1u, w.nodes,
+                                                w.run_ovf, lacc);
+                add_checked(w.run, task_count, w.run_ovf);
             }
             __syncthreads();
+            publish_local<LOCAL>(a, g, task, list, lacc, R, task_count);
+            if (LOCAL) __syncthreads();
             const uint64_t dt = wall_clock64() - t0;
             w.tmax = dt > w.tmax ? dt : w.tmax;
             w.tsum += dt;
@@ -923,12 +1188,18 @@ __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t
 }
 
 // ---- host -------------------------------------------------------------------
-static const void *generic_kernel(bool pivot, bool global)
+static const void *generic_kernel(bool pivot, bool global, bool local)
 {
-    if (pivot) return global ? reinterpret_cast<const void *>(&k_count<true, true>)
-                             : reinterpret_cast<const void *>(&k_count<true, false>);
-    return global ? reinterpret_cast<const void *>(&k_count<false, true>)
-                  : reinterpret_cast<const void *>(&k_count<false, false>);
+    static const void *const k[8] = {
+        reinterpret_cast<const void *>(&k_count<false, false, false>),
+        reinterpret_cast<const void *>(&k_count<false, true, false>),
+        reinterpret_cast<const void *>(&k_count<true, false, false>),
+        reinterpret_cast<const void *>(&k_count<true, true, false>),
+        reinterpret_cast<const void *>(&k_count<false, false, true>),
+        reinterpret_cast<const void *>(&k_count<false, true, true>),
+        reinterpret_cast<const void *>(&k_count<true, false, true>),
+        reinterpret_cast<const void *>(&k_count<true, true, true>)};
+    return k[(local ? 4 : 0) + (pivot ? 2 : 0) + (global ? 1 : 0)];
 }
 
 static const void *register_kernel(int c)
@@ -957,7 +1228,7 @@ static void binom_table(uint32_t nmax, uint32_t rmax, std::vector<uint64_t> &val
 }
 
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
-                LaunchStats *ls)
+                LaunchStats *ls, uint64_t *local)
 {
     DeviceCtx *ctx;
     int rc = device_ctx(&ctx);
@@ -965,6 +1236,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     const uint32_t nq = (uint32_t)qs.size();
     res.assign(nq, kc_result{});
     if (nq == 0) return KC_OK;
+    if (local && nq != 1) return fail(KC_ERR_ARG, "per-vertex counts take one query per launch");
     constexpr int NB = 2 * NCLASS;  // (algo, size class) bins
 
     // --- query descriptors and the concatenated task space ---
@@ -1079,24 +1351,26 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         a.acc = dacc + ACC * b;
         a.qcount = qcount;
         a.qovf = qovf;
+        a.local = local;
+        const bool lc = local != nullptr;
         bool global = (c == NCLASS - 1);
         const uint32_t rows_cap = global ? bmax[b] : CLASS_ROWS[c];
         a.rmax = rows_cap;
         a.wmax = (rows_cap + 63) / 64;
         a.levels = pivot ? kmax + 3 : std::max<uint32_t>(1, kmax - 2);
         uint32_t np = 64 / P;
-        Layout L = make_layout(a.rmax, a.wmax, a.levels, np, pivot);
+        Layout L = make_layout(a.rmax, a.wmax, a.levels, np, pivot, lc);
         if (!global && !pivot) {
             while (np > 1 && L.total * 8 > lds_cap) {  // fewer partitions when the stacks do not fit
                 np >>= 1;
-                L = make_layout(a.rmax, a.wmax, a.levels, np, pivot);
+                L = make_layout(a.rmax, a.wmax, a.levels, np, pivot, lc);
             }
         }
         if (!global && L.total * 8 > lds_cap) global = true;  // same layout in HBM
         a.nparts = np;
-        const void *kern = generic_kernel(pivot, global);
+        const void *kern = generic_kernel(pivot, global, lc);
         variant[b] = global ? 3u : 1u;
-        if (pivot && !global && c <= 2) {
+        if (pivot && !global && c <= 2 && !lc) {
             kern = register_kernel(c);
             L = make_layout(a.rmax, a.wmax, 0, 0, true);
             variant[b] = 2u;
@@ -1227,6 +1501,26 @@ kc_query kc::tune_query(const kc_oriented *o, const kc_query *q)
     return t;
 }
 
+static void fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls)
+{
+    *r = res;
+    r->tasks = ls.tasks;
+    r->alg_bytes = ls.alg_bytes;
+    r->tree_nodes = ls.tree_nodes;
+    r->count_ms = ls.count_ms;
+    r->dominant_ms = ls.dominant_ms;
+    r->dominant_bytes = ls.dominant_bytes;
+    r->dominant_class = ls.dominant_class;
+    for (int c = 0; c < 6; ++c) {
+        r->class_ms[c] = ls.class_ms[c];
+        r->class_bytes[c] = ls.class_bytes[c];
+        r->class_tasks[c] = ls.class_tasks[c];
+        r->class_kernel[c] = ls.class_kernel[c];
+        r->class_max_task_ms[c] = ls.class_max_task_ms[c];
+        r->class_task_ms_sum[c] = ls.class_task_ms_sum[c];
+    }
+}
+
 extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
 {
     if (!o || !q || !r) return fail(KC_ERR_ARG, "kc_count: null argument");
@@ -1250,22 +1544,46 @@ extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
     LaunchStats ls;
     rc = run_queries(om->scr, om->stream, qs, res, &ls);
     if (rc) return rc;
-    *r = res[0];
-    r->tasks = ls.tasks;
-    r->alg_bytes = ls.alg_bytes;
-    r->tree_nodes = ls.tree_nodes;
-    r->count_ms = ls.count_ms;
-    r->dominant_ms = ls.dominant_ms;
-    r->dominant_bytes = ls.dominant_bytes;
-    r->dominant_class = ls.dominant_class;
-    for (int c = 0; c < 6; ++c) {
-        r->class_ms[c] = ls.class_ms[c];
-        r->class_bytes[c] = ls.class_bytes[c];
-        r->class_tasks[c] = ls.class_tasks[c];
-        r->class_kernel[c] = ls.class_kernel[c];
-        r->class_max_task_ms[c] = ls.class_max_task_ms[c];
-        r->class_task_ms_sum[c] = ls.class_task_ms_sum[c];
+    fill_result(r, res[0], ls);
+    return r->overflow ? fail(KC_ERR_OVERFLOW, "k-clique count exceeds 2^64-1") : KC_OK;
+}
+
+extern "C" int kc_count_local(const kc_oriented *o, const kc_query *q, uint64_t *local, kc_result *r)
+{
+    if (!o || !q || !r || (!local && o->n)) return fail(KC_ERR_ARG, "kc_count_local: null argument");
+    int rc = check_query(q);
+    if (rc) return rc;
+    hipSetDevice(o->dev);
+    kc_query qq = tune(o, q);
+    std::memset(r, 0, sizeof *r);
+    r->algo = qq.algo;
+    r->gran = qq.gran;
+    r->part_size = qq.part_size;
+    const uint32_t n = o->n;
+    if (qq.k <= 2) {
+        // every vertex is in one 1-clique; in deg(v) = out + in 2-cliques
+        std::vector<uint32_t> rp((size_t)n + 1), col(o->m);
+        if (qq.k == 2 && (rc = kc_oriented_export(o, rp.data(), col.data(), nullptr))) return rc;
+        for (uint32_t v = 0; v < n; ++v) local[v] = qq.k == 1 ? 1 : rp[v + 1] - rp[v];
+        if (qq.k == 2)
+            for (uint32_t x : col) local[x]++;
+        r->count = qq.k == 1 ? n : o->m;
+        r->tasks = qq.gran == KC_GRAN_EDGE ? o->m : o->n;
+        return KC_OK;
     }
+    kc_oriented *om = const_cast<kc_oriented *>(o);
+    if ((rc = om->scr.local.ensure(sizeof(uint64_t) * ((size_t)n + 1)))) return rc;
+    KC_HIP(hipMemsetAsync(om->scr.local.p, 0, sizeof(uint64_t) * (size_t)n, om->stream));
+    std::vector<QueryRun> qs{QueryRun{o, qq}};
+    std::vector<kc_result> res;
+    LaunchStats ls;
+    rc = run_queries(om->scr, om->stream, qs, res, &ls, om->scr.local.as<uint64_t>());
+    if (rc) return rc;
+    if (n) {
+        KC_HIP(hipMemcpyAsync(local, om->scr.local.p, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToHost, om->stream));
+        KC_HIP(hipStreamSynchronize(om->stream));
+    }
+    fill_result(r, res[0], ls);
     return r->overflow ? fail(KC_ERR_OVERFLOW, "k-clique count exceeds 2^64-1") : KC_OK;
 }
 

@@ -46,7 +46,7 @@ int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tm
 
 // scratch of one launch sequence (grow-only)
 struct Scratch {
-    DevBuf tasks, meta, binom, gscr, size, qdesc;
+    DevBuf tasks, meta, binom, gscr, size, qdesc, local;
 };
 
 struct QueryRun {
@@ -64,9 +64,11 @@ struct LaunchStats {
 };
 
 // Count every query of qs in persistent launches on stream s (all graphs on
-// the current device): one launch per (algorithm, size class) bin.
+// the current device): one launch per (algorithm, size class) bin.  With
+// local != null (one query only) the per-vertex counts are added into
+// local[0..n) on the device as well.
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
-                LaunchStats *ls);
+                LaunchStats *ls, uint64_t *local = nullptr);
 
 // query validation and the autotuner's choice of algo / gran / P for a query
 int check_query(const kc_query *q);

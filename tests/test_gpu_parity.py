@@ -345,3 +345,70 @@ def test_oriented_cache_round_trip(tmp_path, golden):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert f"kcliques={g['counts'][6]} k=6 overflow=0" in r.stdout
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("gran", GRANS)
+def test_local_counts_vs_golden(golden, algo, gran):
+    """-m lkc: per-vertex counts equal networkx's for every vertex."""
+    for g in golden:
+        if algo == "orient" and g["name"] == "K25":
+            continue
+        G = _graph(g)
+        for mode in ("degree", "degen"):
+            Og = G.orient(mode)
+            for k in (1, 2, 3, 4, 5, 6):
+                for P in (1, 8):
+                    loc, r = Og.count_local(k, algo, gran, part_size=P)
+                    if k <= 2:
+                        exp = [1] * g["n"] if k == 1 else np.bincount(
+                            np.array(g["edges"], dtype=np.int64).ravel(), minlength=g["n"]).tolist()
+                    else:
+                        exp = g["local"][str(k)]
+                    assert loc.tolist() == exp, (g["name"], mode, k, P)
+                    assert r.count == g["counts"][k]
+
+
+@pytest.mark.parametrize("mode", ["degree", "degen"])
+def test_local_counts_size_classes_vs_oracle(mode):
+    G = _planted(6000, 30000, [80, 150, 300, 600, 1100], seed=5)
+    rp, col = _host_csr(G)
+    key = O.peel_priority(rp, col)[0] if mode == "degen" else None
+    orp, oc = O.orient(rp, col, key)
+    Og = G.orient(mode)
+    # (orientation credits every clique member one by one: C(1100, 4) adds at
+    # k=4 would take minutes, so it runs at k=3 here)
+    for k, algos in ((3, ALGOS), (4, ("pivot",)), (7, ("pivot",))):
+        exp, ovf = O.count_local(orp, oc, k, "pivot_tails", "node")
+        for algo in algos:
+            for gran in GRANS:
+                loc, r = Og.count_local(k, algo, gran, part_size=8)
+                print(mode, k, algo, gran, r.raw["count_ms"], flush=True)
+                assert (loc == exp).all() and not r.overflow, (mode, k, algo, gran)
+    # a power-law graph: every vertex against the oracle, sum = k * count
+    G = K.Graph.generate(K.GEN_CHUNG_LU, seed=11, n=30000, m=200000, gamma=2.1, max_degree=1500)
+    rp, col = _host_csr(G)
+    orp, oc = O.orient(rp, col, O.peel_priority(rp, col)[0])
+    Og = G.orient("degen")
+    for k in (5, 8):
+        exp, _ = O.count_local(orp, oc, k, "pivot_tails", "edge")
+        loc, r = Og.count_local(k, "pivot", "edge", part_size=1)
+        assert (loc == exp).all()
+        assert int(loc.sum(dtype=np.uint64)) == k * r.count == k * Og.count(k, "pivot", "edge").count
+
+
+def test_local_counts_cli(tmp_path, golden):
+    import os
+    import re
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kcgpu_amd", "bin", "kcgpu")
+    g = next(x for x in golden if x["name"] == "planted_200")
+    p = tmp_path / "g.txt"
+    p.write_text("".join(f"{b} {a}\n" for a, b in g["edges"]))  # "dst src", 0-based
+    r = subprocess.run([cli, "-g", str(p), "-m", "lkc", "-o", "degen", "-k", "5", "-p", "node", "-q", "p8b"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Local clique counter for the first 200 Nodes:" in r.stdout
+    rows = re.findall(r"^(\d+)\t(\d+)$", r.stdout, flags=re.M)
+    assert [int(c) for _, c in rows] == g["local"]["5"][:50]
+    assert f"kcliques={g['counts'][5]} k=5 overflow=0" in r.stdout

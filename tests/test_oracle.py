@@ -173,3 +173,52 @@ def test_planted_mistake_is_caught(golden):
     rp, col = _csr(g)
     r = O.count(rp.astype(np.uint64), col, 4, "orient", "node")  # unoriented CSR
     assert r["count"] != g["counts"][4]
+
+
+# ---- local (per-vertex) counts, -m lkc ------------------------------------------
+@pytest.mark.parametrize("algo", ["orient", "pivot", "pivot_tails"])
+@pytest.mark.parametrize("gran", ["node", "edge"])
+def test_local_counts_vs_golden(golden, algo, gran):
+    for g in golden:
+        if algo == "orient" and g["name"] == "K25":
+            continue
+        rp, col = _csr(g)
+        pr, _ = O.peel_priority(rp, col)
+        for key in (None, pr):
+            orp, oc = O.orient(rp, col, key)
+            for k in (3, 4, 5, 6):
+                loc, ovf = O.count_local(orp, oc, k, algo, gran)
+                assert not ovf
+                assert loc.tolist() == g["local"][str(k)], (g["name"], k, algo, gran)
+
+
+def test_local_counts_closed_forms_and_invariants(golden):
+    # K_n: every vertex lies in C(n-1, k-1) k-cliques; k=1 -> 1; k=2 -> degree
+    n = 30
+    src, dst = zip(*[(i, j) for i in range(n) for j in range(i + 1, n)])
+    rp, col = O.csr_from_edges(src, dst, n)
+    orp, oc = O.orient(rp, col)
+    for k in (1, 2, 3, 7, 12):
+        for algo in ("pivot", "pivot_tails"):
+            for gran in ("node", "edge"):
+                loc, ovf = O.count_local(orp, oc, k, algo, gran)
+                assert (loc == comb(n - 1, k - 1)).all() and not ovf, (k, algo, gran)
+    # sum over vertices = k * count, for counts beyond the golden k range
+    g = next(x for x in golden if x["name"] == "gnp_150_0.4")
+    rp, col = _csr(g)
+    pr, _ = O.peel_priority(rp, col)
+    orp, oc = O.orient(rp, col, pr)
+    for k in (7, 8, 9):
+        for gran in ("node", "edge"):
+            loc, _ = O.count_local(orp, oc, k, "pivot_tails", gran)
+            assert int(loc.sum(dtype=np.uint64)) == k * g["counts"][k]
+            assert (loc == O.count_local(orp, oc, k, "pivot", gran)[0]).all()
+    # overflow of one vertex's count is flagged: K70, k = 31 -> C(69, 30) > 2^64
+    n = 70
+    src, dst = zip(*[(i, j) for i in range(n) for j in range(i + 1, n)])
+    rp, col = O.csr_from_edges(src, dst, n)
+    orp, oc = O.orient(rp, col)
+    assert O.count_local(orp, oc, 31, "pivot_tails", "node")[1]
+    loc, ovf = O.count_local(orp, oc, 10, "pivot_tails", "edge")
+    assert (loc == comb(69, 9)).all() and not ovf
+
