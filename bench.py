@@ -83,8 +83,13 @@ def run_batch(args):
         made = list(ex.map(make, specs))
     log(f"generated {len(made)} graphs in {time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    ors = [G.orient(plan["orient"]) for G, plan in made]
-    log(f"oriented in {time.perf_counter() - t:.1f}s")
+    ors_plain = [G.orient(plan["orient"]) for G, plan in made]
+    t_orient_plain = time.perf_counter() - t
+    del ors_plain
+    t = time.perf_counter()
+    ors = K.orient_batch([G for G, _ in made], [plan["orient"] for _, plan in made])
+    t_orient_batch = time.perf_counter() - t
+    log(f"oriented one by one in {t_orient_plain:.2f}s, batched in {t_orient_batch:.2f}s")
     m_total = sum(o.info()["m"] for o in ors)
     queries = [dict(k=sp["k"], algo=plan["algo"], gran=plan["gran"], part_size=plan["part_size"])
                for sp, (G, plan) in zip(specs, made)]
@@ -108,6 +113,8 @@ def run_batch(args):
                                   f"n~U[200,20000] avg degree~U[4,40], seed(i)=splitmix64(4^i), k~U{{4..7}}, autotuned",
                       "m_oriented_total": m_total, "plain_s": t_plain, "batched_s": t_batch,
                       "speedup_vs_plain": t_plain / t_batch, "plain_queries_per_s": len(ors) / t_plain,
+                      "orient_plain_s": t_orient_plain, "orient_batched_s": t_orient_batch,
+                      "orient_batched_device_ms": ors[0].info()["preprocess_ms"] if ors else 0.0,
                       "batched_edges_per_s": m_total / t_batch, "device_ms": st["total_ms"],
                       "tasks": st["tasks"],
                       "plans": {x: sum(1 for _, p in made if p["orient"] == x) for x in ("degree", "degen")}}}

@@ -155,6 +155,19 @@ typedef struct {
  * The oriented CSR stays in HBM inside the returned handle.
  */
 int kc_orient(const kc_graph *g, int mode, kc_oriented **out);
+
+/*
+ * Orient nb graphs, graph i by modes[i]; out[i] is what kc_orient(gs[i],
+ * modes[i]) returns (same CSR, key and max out-degree).  Degree-order graphs
+ * are preprocessed together: one upload of the concatenated CSRs and one
+ * captured hipGraph of the orientation kernels over all of them, with the
+ * oriented CSRs in one shared allocation (freed with the last of those
+ * handles); preprocess_ms of each is the whole batch's device time.
+ * Degeneracy-order graphs are oriented one by one.  On error no handle is
+ * returned.  No reference counterpart (the reference reads one graph per
+ * process, src/main.cu:73-235).
+ */
+int kc_orient_batch(const kc_graph *const *gs, const int *modes, size_t nb, kc_oriented **out);
 int kc_oriented_get_info(const kc_oriented *o, kc_oriented_info *info);
 /* Copy the oriented CSR (rowptr[n+1], col[m]) and the orientation key
  * (degree or peel round, key[n]) to the host; any pointer may be NULL. */

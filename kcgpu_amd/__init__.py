@@ -124,6 +124,8 @@ SIGNATURES = {
     "kc_graph_free": (None, [_vp]),
     "kc_generate": (ctypes.c_int, [ctypes.POINTER(GenParams), ctypes.POINTER(_vp)]),
     "kc_orient": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "kc_orient_batch": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
+                                       ctypes.POINTER(_vp)]),
     "kc_oriented_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(OrientedInfo)]),
     "kc_oriented_export": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
     "kc_oriented_free": (None, [_vp]),
@@ -246,6 +248,16 @@ class Graph:
         h = ctypes.c_void_p()
         _check(lib().kc_orient(self._h, ORIENT[mode] if isinstance(mode, str) else int(mode), ctypes.byref(h)))
         return Oriented(h.value)
+
+
+def orient_batch(graphs, modes):
+    """Orient many graphs at once (kc_orient_batch): list of Oriented."""
+    n = len(graphs)
+    gs = (_vp * max(n, 1))(*[g._h.value for g in graphs])
+    ms = (ctypes.c_int * max(n, 1))(*[ORIENT[m] if isinstance(m, str) else int(m) for m in modes])
+    out = (_vp * max(n, 1))()
+    _check(lib().kc_orient_batch(gs, ms, n, out))
+    return [Oriented(out[i]) for i in range(n)]
 
 
 def read_edges_reference(path):

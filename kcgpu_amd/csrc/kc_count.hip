@@ -352,218 +352,10 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
     return cnt;
 }
 
-// ---- step 3b: pivoting (SCT) ----------------------------------------------
-struct Scal {
-    uint32_t held, piv, pivot, pad;
-};
-
-// A subtree rooted at a node with h held vertices, p pivot vertices and
-// candidate set X represents every clique R_h + S + Q with S a clique of G[X]
-// and Q a subset of the pivots, so its k-cliques number
-//     sum_j C(p, k-h-j) * c_j(X),   c_j = j-cliques of G[X].
-// With c_0 = 1, c_1 = |X| and c_2 = |E(X)| = (1/2) sum_{j in X} |X & row_j|
-// (the pivot scores) the subtree is closed-form once h >= k-2:
-//     h = k:   1        h = k-1:   p + |X|        h = k-2:  C(p,2) + p|X| + |E(X)|
-// so the SCT is only expanded while h < k-2.  The register-bitmap kernel also
-// closes h = k-3 with c_3 = triangles of G[X].  A node is also dropped when
-// h + p + max_j|X & row_j| + 1 < k (the largest clique inside X has at most
-// maxIntersection + 1 vertices, cf. kclique/kckernels.cuh:2573-2584).
-//
-// LOCAL also credits every task vertex in lacc with the k-cliques of the
-// subtree that contain it (the root with the task's count, by the caller).
-// A subtree (H, P, X) with t = k - h holds the cliques H + S + Q, S a subset
-// of P and Q a clique of G[X] with |S| + |Q| = t, so
-//     held vertex  sum_j C(p, t-j) c_j(X)      pivot vertex  sum_j C(p-1, t-1-j) c_j(X)
-//     x in X       sum_j C(p, t-j) c_{j-1}(X & row_x)
-// which at a leaf (X empty) is the reference's hold/pivot split
-// (kclique/kclique_local.cuh, kckernels_local.cuh).  The sets H and P of the
-// node live beside X, and each stack level keeps its own copies.  Vertex sets
-// H, P and X are disjoint and lane w owns the 64 vertices of word w, so the
-// adds need no atomics; no add can wrap unless the task count does.
-__device__ inline uint64_t binom_at(const CountArgs &a, int64_t n, int64_t r)
-{
-    return (r < 0 || n < 0 || r > n) ? 0ull : a.binom[(uint64_t)n * a.bstride + (uint64_t)r];
-}
-
-template <bool LOCAL>
-__device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
-                                Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
-                                uint32_t &ovf, uint64_t *lacc)
-{
-    const int lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    const uint32_t wmax = a.wmax;
-    const uint32_t P = a.psize, NPp = 64 / P, part = lane / P, lip = lane % P;
-    const uint32_t lvl = LOCAL ? 4 * wmax : 2 * wmax;  // stack words per level
-    uint64_t *Hc = X + wmax, *Pc = X + 2 * wmax;     // LOCAL: held / pivot sets of the node
-    uint64_t acc = 0;
-    // root candidate set = all R list vertices
-    for (uint32_t w = lane; w < W; w += 64) {
-        uint32_t lo = w * 64;
-        X[w] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
-        if (LOCAL) Hc[w] = Pc[w] = 0;
-    }
-    __syncthreads();
-    int t = -1;
-    bool pending = true;
-    uint32_t h = h0, p = 0;
-    uint64_t my_nodes = 0;
-    // LOCAL: add vh to each held and vp to each pivot vertex of the node
-    auto credit_hp = [&](uint64_t vh, uint64_t vp) {
-        if (!LOCAL || (!vh && !vp)) return;
-        for (uint32_t w = lane; w < W; w += 64) {
-            if (vh)
-                for (uint64_t x = Hc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vh;
-            if (vp)
-                for (uint64_t x = Pc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vp;
-        }
-    };
-    for (;;) {
-        if (pending) {
-            pending = false;
-            ++my_nodes;
-            // candidate list of X (uniform word loop: the ballot of a word is the word)
-            uint32_t cx = 0;
-            for (uint32_t w = 0; w < W; ++w) {
-                const uint64_t xw = X[w];
-                if ((xw >> lane) & 1ull) cand[cx + (uint32_t)__popcll(xw & lt)] = w * 64 + (uint32_t)lane;
-                cx += (uint32_t)__popcll(xw);
-            }
-            __syncthreads();
-            if (cx == 0) {
-                if (k >= h && k - h <= p) {
-                    uint64_t idx = (uint64_t)p * a.bstride + (k - h);
-                    if (a.binom_ovf[idx]) ovf = 1;
-                    add_checked(acc, a.binom[idx], ovf);
-                    if (LOCAL) credit_hp(a.binom[idx], binom_at(a, (int64_t)p - 1, (int64_t)(k - h) - 1));
-                }
-            } else if (h >= k) {
-                add_checked(acc, 1, ovf);
-                if (LOCAL) credit_hp(1, 0);
-            } else if (h + 1 == k) {
-                add_checked(acc, (uint64_t)p + cx, ovf);
-                if (LOCAL) {
-                    credit_hp((uint64_t)p + cx, 1);
-                    for (uint32_t c = lane; c < cx; c += 64) lacc[cand[c]] += 1;
-                }
-            } else if (h + p + cx >= k) {
-                // scores |X & row_j| for j in X: P lanes per candidate
-                uint32_t best_s = 0, best_j = INF32;
-                uint64_t sum_s = 0;
-                for (uint32_t c = part; c < cx; c += NPp) {
-                    const uint32_t j = cand[c];
-                    const uint64_t *rj = rows + (uint64_t)j * W;
-                    uint32_t sc = 0;
-                    for (uint32_t w = lip; w < W; w += P) sc += (uint32_t)__popcll(X[w] & rj[w]);
-                    sc = part_sum32(sc, P);
-                    sum_s += sc;
-                    if (best_j == INF32 || sc > best_s) {
-                        best_s = sc;
-                        best_j = j;
-                    }
-                    // h = k-2: x in X lies in C(p,1) + |X & row_x| of the subtree's cliques
-                    if (LOCAL && h + 2 == k && lip == 0) lacc[j] += (uint64_t)p + sc;
-                }
-                uint64_t key = (lip == 0 && best_j != INF32) ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
-                key = wave_max64(key);
-                uint32_t dummy = 0;
-                const uint64_t tot_s = wave_sum64_checked(lip == 0 ? sum_s : 0ull, dummy);
-                const uint32_t max_s = (uint32_t)(key >> 32);
-                if (h + 2 == k) {
-                    const uint64_t before = acc;
-                    add_checked(acc, (uint64_t)p * (p - (p ? 1 : 0)) / 2, ovf);
-                    add_checked(acc, (uint64_t)p * cx, ovf);
-                    add_checked(acc, tot_s / 2, ovf);
-                    if (LOCAL) credit_hp(acc - before, (p ? (uint64_t)p - 1 : 0) + cx);
-                } else if (h + p + max_s + 1 >= k) {
-                    const uint32_t pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
-                    ++t;
-                    uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
-                    const uint64_t *rp = rows + (uint64_t)pv * W;
-                    for (uint32_t w = lane; w < W; w += 64) {
-                        uint64_t x = X[w];
-                        Cc[w] = x;
-                        B[w] = x & ~rp[w];
-                        if (LOCAL) {
-                            B[2 * wmax + w] = Hc[w];
-                            B[3 * wmax + w] = Pc[w];
-                        }
-                    }
-                    if (lane == 0) scal[t] = Scal{h, p, pv, 0};
-                    __syncthreads();
-                }
-            }
-        }
-        if (t < 0) break;
-        // next branch of level t: non-pivot vertices first, the pivot last
-        uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
-        const Scal sc = scal[t];
-        uint32_t b = INF32;
-        for (uint32_t w = lane; w < W; w += 64) {
-            uint64_t x = B[w];
-            if (w == (sc.pivot >> 6)) x &= ~(1ull << (sc.pivot & 63));
-            if (x) {
-                b = w * 64 + (uint32_t)__builtin_ctzll(x);
-                break;
-            }
-        }
-        b = wave_min32(b);
-        const bool last = (b == INF32);
-        if (last) b = sc.pivot;
-        const uint64_t *rb = rows + (uint64_t)b * W;
-        for (uint32_t w = lane; w < W; w += 64) {
-            uint64_t c = Cc[w];
-            X[w] = c & rb[w];
-            if (LOCAL) {
-                const uint64_t bit = (w == (b >> 6)) ? (1ull << (b & 63)) : 0ull;
-                Hc[w] = B[2 * wmax + w] | (b != sc.pivot ? bit : 0ull);
-                Pc[w] = B[3 * wmax + w] | (b == sc.pivot ? bit : 0ull);
-            }
-            if (w == (b >> 6)) {
-                Cc[w] = c & ~(1ull << (b & 63));
-                B[w] &= ~(1ull << (b & 63));
-            }
-        }
-        __syncthreads();
-        h = sc.held + (b != sc.pivot ? 1u : 0u);
-        p = sc.piv + (b == sc.pivot ? 1u : 0u);
-        if (last) --t;  // the pivot branch replaces its level
-        if (h <= k) pending = true;
-    }
-    if (lane == 0) nodes += my_nodes;
-    return acc;
-}
-
-// ---- step 3c: pivoting with the bitmap in registers (R <= 64*S) -------------
-// Lane l holds rows l, l+64, ..., l+64(S-1) of the full bitmap in VGPRs, so a
-// pivot score is S*S AND+popcount on the lane's own registers; the rows a
-// branch needs whole are broadcast-read from the LDS copy left by the build.  The SCT
-// state is uniform: candidate set X and the level variables live in scalar
-// registers, and the stack is spread across the lanes (lane t keeps level t in
-// VGPRs, read back with v_readlane).  Arg-max and sums use ballots: the max
-// by a bitwise search, the sum bit-sliced, so no LDS and no shuffles are
-// touched after the rows are loaded.
 __device__ inline uint64_t rdlane64(uint64_t v, uint32_t src)
 {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
     uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ inline uint32_t ballot_sum(uint32_t v, int bits)
-{
-    uint32_t s = 0;
-    for (int b = 0; b < bits; ++b) s += (uint32_t)__popcll(__ballot((v >> b) & 1u)) << b;
-    return s;
-}
-
-// word w of row b from the LDS copy (uniform address: a broadcast read)
-__device__ inline uint64_t lds_row_word(const uint64_t *rows, uint32_t W, uint32_t b, int w)
-{
-    if ((uint32_t)w >= W) return 0ull;
-    const uint64_t v = rows[(uint64_t)b * W + w];
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -664,6 +456,229 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
     }
     if (lane == 0) nodes += my_nodes;
     return acc;
+}
+
+// ---- step 3b: pivoting (SCT) ----------------------------------------------
+struct Scal {
+    uint32_t held, piv, pivot, pad;
+};
+
+// A subtree rooted at a node with h held vertices, p pivot vertices and
+// candidate set X represents every clique R_h + S + Q with S a clique of G[X]
+// and Q a subset of the pivots, so its k-cliques number
+//     sum_j C(p, k-h-j) * c_j(X),   c_j = j-cliques of G[X].
+// With c_0 = 1, c_1 = |X| and c_2 = |E(X)| = (1/2) sum_{j in X} |X & row_j|
+// (the pivot scores) the subtree is closed-form once h >= k-2:
+//     h = k:   1        h = k-1:   p + |X|        h = k-2:  C(p,2) + p|X| + |E(X)|
+// so the SCT is only expanded while h < k-2.  The register-bitmap kernel also
+// closes h = k-3 with c_3 = triangles of G[X].  A node is also dropped when
+// h + p + max_j|X & row_j| + 1 < k (the largest clique inside X has at most
+// maxIntersection + 1 vertices, cf. kclique/kckernels.cuh:2573-2584).
+//
+// LOCAL also credits every task vertex in lacc with the k-cliques of the
+// subtree that contain it (the root with the task's count, by the caller).
+// A subtree (H, P, X) with t = k - h holds the cliques H + S + Q, S a subset
+// of P and Q a clique of G[X] with |S| + |Q| = t, so
+//     held vertex  sum_j C(p, t-j) c_j(X)      pivot vertex  sum_j C(p-1, t-1-j) c_j(X)
+//     x in X       sum_j C(p, t-j) c_{j-1}(X & row_x)
+// which at a leaf (X empty) is the reference's hold/pivot split
+// (kclique/kclique_local.cuh, kckernels_local.cuh).  The sets H and P of the
+// node live beside X, and each stack level keeps its own copies.  Vertex sets
+// H, P and X are disjoint and lane w owns the 64 vertices of word w, so the
+// adds need no atomics; no add can wrap unless the task count does.
+__device__ inline uint64_t binom_at(const CountArgs &a, int64_t n, int64_t r)
+{
+    return (r < 0 || n < 0 || r > n) ? 0ull : a.binom[(uint64_t)n * a.bstride + (uint64_t)r];
+}
+
+template <bool LOCAL>
+__device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
+                                Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
+                                uint32_t &ovf, uint64_t *lacc)
+{
+    const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const uint32_t wmax = a.wmax;
+    const uint32_t P = a.psize, NPp = 64 / P, part = lane / P, lip = lane % P;
+    const uint32_t lvl = LOCAL ? 4 * wmax : 2 * wmax;  // stack words per level
+    uint64_t *Hc = X + wmax, *Pc = X + 2 * wmax;     // LOCAL: held / pivot sets of the node
+    uint64_t acc = 0;
+    // root candidate set = all R list vertices
+    for (uint32_t w = lane; w < W; w += 64) {
+        uint32_t lo = w * 64;
+        X[w] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
+        if (LOCAL) Hc[w] = Pc[w] = 0;
+    }
+    __syncthreads();
+    int t = -1;
+    bool pending = true;
+    uint32_t h = h0, p = 0;
+    uint64_t my_nodes = 0;
+    // LOCAL: add vh to each held and vp to each pivot vertex of the node
+    auto credit_hp = [&](uint64_t vh, uint64_t vp) {
+        if (!LOCAL || (!vh && !vp)) return;
+        for (uint32_t w = lane; w < W; w += 64) {
+            if (vh)
+                for (uint64_t x = Hc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vh;
+            if (vp)
+                for (uint64_t x = Pc[w]; x; x &= x - 1) lacc[w * 64 + (uint32_t)__builtin_ctzll(x)] += vp;
+        }
+    };
+    for (;;) {
+        if (pending) {
+            pending = false;
+            ++my_nodes;
+            // candidate list of X (uniform word loop: the ballot of a word is the word)
+            uint32_t cx = 0;
+            for (uint32_t w = 0; w < W; ++w) {
+                const uint64_t xw = X[w];
+                if ((xw >> lane) & 1ull) cand[cx + (uint32_t)__popcll(xw & lt)] = w * 64 + (uint32_t)lane;
+                cx += (uint32_t)__popcll(xw);
+            }
+            __syncthreads();
+            if (cx == 0) {
+                if (k >= h && k - h <= p) {
+                    uint64_t idx = (uint64_t)p * a.bstride + (k - h);
+                    if (a.binom_ovf[idx]) ovf = 1;
+                    add_checked(acc, a.binom[idx], ovf);
+                    if (LOCAL) credit_hp(a.binom[idx], binom_at(a, (int64_t)p - 1, (int64_t)(k - h) - 1));
+                }
+            } else if (h >= k) {
+                add_checked(acc, 1, ovf);
+                if (LOCAL) credit_hp(1, 0);
+            } else if (h + 1 == k) {
+                add_checked(acc, (uint64_t)p + cx, ovf);
+                if (LOCAL) {
+                    credit_hp((uint64_t)p + cx, 1);
+                    for (uint32_t c = lane; c < cx; c += 64) lacc[cand[c]] += 1;
+                }
+            } else if (!LOCAL && cx <= 64 && h + 3 <= k && h + p + cx >= k) {
+                // at most 64 candidates left: re-encode the subtree one word per
+                // row (row j of the subproblem = ballot of bit cand[j] over the
+                // rows of the candidates) and finish it in sct64
+                const uint32_t mine = lane < cx ? cand[lane] : 0u;
+                uint64_t row1 = 0;
+                for (uint32_t j = 0; j < cx; ++j) {
+                    const uint32_t xj = cand[j];
+                    const uint64_t wv = lane < cx ? rows[(uint64_t)mine * W + (xj >> 6)] : 0ull;
+                    const uint64_t col = __ballot((wv >> (xj & 63)) & 1ull);
+                    row1 = (lane == j) ? col : row1;
+                }
+                add_checked(acc, sct64(a, k, row1, cx >= 64 ? ~0ull : ((1ull << cx) - 1), h, p, nodes, ovf), ovf);
+                --my_nodes;  // counted again as the subtree's root
+            } else if (h + p + cx >= k) {
+                // scores |X & row_j| for j in X: P lanes per candidate
+                uint32_t best_s = 0, best_j = INF32;
+                uint64_t sum_s = 0;
+                for (uint32_t c = part; c < cx; c += NPp) {
+                    const uint32_t j = cand[c];
+                    const uint64_t *rj = rows + (uint64_t)j * W;
+                    uint32_t sc = 0;
+                    for (uint32_t w = lip; w < W; w += P) sc += (uint32_t)__popcll(X[w] & rj[w]);
+                    sc = part_sum32(sc, P);
+                    sum_s += sc;
+                    if (best_j == INF32 || sc > best_s) {
+                        best_s = sc;
+                        best_j = j;
+                    }
+                    // h = k-2: x in X lies in C(p,1) + |X & row_x| of the subtree's cliques
+                    if (LOCAL && h + 2 == k && lip == 0) lacc[j] += (uint64_t)p + sc;
+                }
+                uint64_t key = (lip == 0 && best_j != INF32) ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
+                key = wave_max64(key);
+                uint32_t dummy = 0;
+                const uint64_t tot_s = wave_sum64_checked(lip == 0 ? sum_s : 0ull, dummy);
+                const uint32_t max_s = (uint32_t)(key >> 32);
+                if (h + 2 == k) {
+                    const uint64_t before = acc;
+                    add_checked(acc, (uint64_t)p * (p - (p ? 1 : 0)) / 2, ovf);
+                    add_checked(acc, (uint64_t)p * cx, ovf);
+                    add_checked(acc, tot_s / 2, ovf);
+                    if (LOCAL) credit_hp(acc - before, (p ? (uint64_t)p - 1 : 0) + cx);
+                } else if (h + p + max_s + 1 >= k) {
+                    const uint32_t pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
+                    ++t;
+                    uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
+                    const uint64_t *rp = rows + (uint64_t)pv * W;
+                    for (uint32_t w = lane; w < W; w += 64) {
+                        uint64_t x = X[w];
+                        Cc[w] = x;
+                        B[w] = x & ~rp[w];
+                        if (LOCAL) {
+                            B[2 * wmax + w] = Hc[w];
+                            B[3 * wmax + w] = Pc[w];
+                        }
+                    }
+                    if (lane == 0) scal[t] = Scal{h, p, pv, 0};
+                    __syncthreads();
+                }
+            }
+        }
+        if (t < 0) break;
+        // next branch of level t: non-pivot vertices first, the pivot last
+        uint64_t *Cc = stk + (uint64_t)t * lvl, *B = Cc + wmax;
+        const Scal sc = scal[t];
+        uint32_t b = INF32;
+        for (uint32_t w = lane; w < W; w += 64) {
+            uint64_t x = B[w];
+            if (w == (sc.pivot >> 6)) x &= ~(1ull << (sc.pivot & 63));
+            if (x) {
+                b = w * 64 + (uint32_t)__builtin_ctzll(x);
+                break;
+            }
+        }
+        b = wave_min32(b);
+        const bool last = (b == INF32);
+        if (last) b = sc.pivot;
+        const uint64_t *rb = rows + (uint64_t)b * W;
+        for (uint32_t w = lane; w < W; w += 64) {
+            uint64_t c = Cc[w];
+            X[w] = c & rb[w];
+            if (LOCAL) {
+                const uint64_t bit = (w == (b >> 6)) ? (1ull << (b & 63)) : 0ull;
+                Hc[w] = B[2 * wmax + w] | (b != sc.pivot ? bit : 0ull);
+                Pc[w] = B[3 * wmax + w] | (b == sc.pivot ? bit : 0ull);
+            }
+            if (w == (b >> 6)) {
+                Cc[w] = c & ~(1ull << (b & 63));
+                B[w] &= ~(1ull << (b & 63));
+            }
+        }
+        __syncthreads();
+        h = sc.held + (b != sc.pivot ? 1u : 0u);
+        p = sc.piv + (b == sc.pivot ? 1u : 0u);
+        if (last) --t;  // the pivot branch replaces its level
+        if (h <= k) pending = true;
+    }
+    if (lane == 0) nodes += my_nodes;
+    return acc;
+}
+
+// ---- step 3c: pivoting with the bitmap in registers (R <= 64*S) -------------
+// Lane l holds rows l, l+64, ..., l+64(S-1) of the full bitmap in VGPRs, so a
+// pivot score is S*S AND+popcount on the lane's own registers; the rows a
+// branch needs whole are broadcast-read from the LDS copy left by the build.  The SCT
+// state is uniform: candidate set X and the level variables live in scalar
+// registers, and the stack is spread across the lanes (lane t keeps level t in
+// VGPRs, read back with v_readlane).  Arg-max and sums use ballots: the max
+// by a bitwise search, the sum bit-sliced, so no LDS and no shuffles are
+// touched after the rows are loaded.
+
+__device__ inline uint32_t ballot_sum(uint32_t v, int bits)
+{
+    uint32_t s = 0;
+    for (int b = 0; b < bits; ++b) s += (uint32_t)__popcll(__ballot((v >> b) & 1u)) << b;
+    return s;
+}
+
+// word w of row b from the LDS copy (uniform address: a broadcast read)
+__device__ inline uint64_t lds_row_word(const uint64_t *rows, uint32_t W, uint32_t b, int w)
+{
+    if ((uint32_t)w >= W) return 0ull;
+    const uint64_t v = rows[(uint64_t)b * W + w];
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Compaction: once a node's candidate set X has at most 64 vertices, its

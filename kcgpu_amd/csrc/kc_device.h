@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "kc_internal.h"
@@ -30,12 +31,20 @@ struct DeviceCtx {
 // Select and check the device (gfx950).  Returns KC_OK or KC_ERR_DEVICE.
 int device_ctx(DeviceCtx **out);
 
-// Grow-only device buffer.
+// Grow-only device buffer, or a view into memory owned elsewhere (owned = false).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     int ensure(size_t want);
     void release();
+    void view(void *q, size_t b)
+    {
+        release();
+        p = q;
+        bytes = b;
+        owned = false;
+    }
     ~DevBuf() { release(); }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -86,6 +95,7 @@ struct kc_oriented {
     int mode = 0;
     double pre_ms = 0;
     kc::DevBuf rowptr, col, src, key;
+    std::shared_ptr<kc::DevBuf> arena;  // owner of rowptr/col/src/key when they are views (batched orientation)
     // count scratch (grow-only, reused across calls on this handle)
     kc::Scratch scr, batch_scr;
     std::vector<uint32_t> h_rowptr_cache;  // filled lazily by host-side autotune

@@ -22,7 +22,9 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <thread>
 
 #include "kc_device.h"
 
@@ -84,8 +86,9 @@ int device_ctx(DeviceCtx **out)
 
 int DevBuf::ensure(size_t want)
 {
-    if (want <= bytes && p) return KC_OK;
+    if (want <= bytes && p && owned) return KC_OK;
     release();
+    owned = true;
     size_t b = want ? want : 256;
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
@@ -99,9 +102,10 @@ int DevBuf::ensure(size_t want)
 
 void DevBuf::release()
 {
-    if (p) hipFree(p);
+    if (p && owned) hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -291,6 +295,118 @@ __global__ void k_max_u32(const uint32_t *a, uint32_t n, uint32_t *out)
         m = max(m, a[i]);
     for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// ---------------------------------------------------------------------------
+// batched degree orientation: many graphs concatenated, one launch per phase.
+// Graph g owns rows [vbase[g], vbase[g+1]) of the concatenation; its full
+// CSR is frow + rbase[g] (n_g + 1 local offsets) and fcol + ebase[g] (local
+// ids); its oriented CSR goes to orow + rbase[g], and to ocol / osrc from
+// scan[vbase[g]] on, where scan is the exclusive scan of all out-degrees.
+// ---------------------------------------------------------------------------
+struct MultiDesc {
+    const uint32_t *vbase;  // [B+1] first row of each graph (global)
+    const uint64_t *ebase;  // [B] first full-CSR entry of each graph
+    uint32_t nb;            // graphs
+    uint32_t nrows;         // total rows
+};
+
+__device__ inline uint32_t graph_of_row(const MultiDesc &d, uint32_t r)
+{
+    uint32_t lo = 0, hi = d.nb - 1;  // last g with vbase[g] <= r
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (d.vbase[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_orient_count_multi(MultiDesc d, const uint32_t *frow, const uint32_t *fcol,
+                                                             uint32_t *key, uint32_t *outdeg)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < d.nrows; r += nw) {
+        const uint32_t g = graph_of_row(d, (uint32_t)r), u = (uint32_t)r - d.vbase[g];
+        const uint32_t *rp = frow + d.vbase[g] + g, *col = fcol + d.ebase[g];
+        const uint32_t b = rp[u], e = rp[u + 1], ku = e - b;
+        uint32_t cnt = 0;
+        for (uint32_t i = b; i < e; i += 64) {
+            bool k = false;
+            if (i + lane < e) {
+                const uint32_t v = col[i + lane];
+                k = keep_edge(ku, u, rp[v + 1] - rp[v], v);
+            }
+            cnt += (uint32_t)__popcll(__ballot(k));
+        }
+        if (lane == 0) {
+            outdeg[r] = cnt;
+            key[r] = ku;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_orient_write_multi(MultiDesc d, const uint32_t *frow, const uint32_t *fcol,
+                                                             const uint32_t *scan, uint32_t *orow, uint32_t *ocol,
+                                                             uint32_t *osrc)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < d.nrows; r += nw) {
+        const uint32_t g = graph_of_row(d, (uint32_t)r), u = (uint32_t)r - d.vbase[g];
+        const uint32_t *rp = frow + d.vbase[g] + g, *col = fcol + d.ebase[g];
+        const uint32_t b = rp[u], e = rp[u + 1], ku = e - b;
+        const uint32_t gstart = scan[d.vbase[g]];
+        uint32_t w = scan[r];
+        if (lane == 0) {
+            orow[d.vbase[g] + g + u] = w - gstart;
+            if (r + 1 == d.vbase[g + 1]) orow[d.vbase[g] + g + u + 1] = scan[r + 1] - gstart;  // the row end
+        }
+        for (uint32_t i = b; i < e; i += 64) {
+            bool k = false;
+            uint32_t v = 0;
+            if (i + lane < e) {
+                v = col[i + lane];
+                k = keep_edge(ku, u, rp[v + 1] - rp[v], v);
+            }
+            const uint64_t bal = __ballot(k);
+            if (k) {
+                const uint32_t at = w + (uint32_t)__popcll(bal & lt);
+                ocol[at] = v;
+                osrc[at] = u;
+            }
+            w += (uint32_t)__popcll(bal);
+        }
+    }
+}
+
+// per-graph max out-degree; a wave whose 64 rows lie in one graph does one atomic
+__global__ __launch_bounds__(256) void k_max_multi(MultiDesc d, const uint32_t *outdeg, uint32_t *maxout)
+{
+    const int lane = threadIdx.x & 63;
+    for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; base < d.nrows;
+         base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = base + lane;
+        const bool in = r < d.nrows;
+        const uint32_t g = in ? graph_of_row(d, (uint32_t)r) : 0xFFFFFFFFu;
+        const uint32_t v = in ? outdeg[r] : 0u;
+        const uint32_t g0 = __shfl(g, 0, 64);
+        if (__ballot(in && g != g0) == 0) {
+            uint32_t m = v;
+            for (int dd = 32; dd >= 1; dd >>= 1) m = max(m, (uint32_t)__shfl_xor(m, dd, 64));
+            if (lane == 0) atomicMax(&maxout[g0], m);
+        } else if (in) {
+            atomicMax(&maxout[g], v);
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -749,6 +865,188 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     o->max_core = max_core;
     o->pre_ms = ms;
     *out = o;
+    return KC_OK;
+}
+
+// Batched orientation.  Degree-order graphs are concatenated into one pinned
+// upload and oriented by one captured hipGraph (max-out reset, row counts,
+// three scan kernels, compaction, per-graph max out-degree); their oriented
+// CSRs are views into one shared allocation.  Degeneracy-order and empty
+// graphs go through kc_orient one by one.
+static int orient_degree_chunk(DeviceCtx *ctx, const kc_graph *const *gs, const std::vector<size_t> &idx,
+                               kc_oriented **out)
+{
+    const size_t B = idx.size();
+    hipStream_t s = ctx->stream;
+    std::vector<uint32_t> vbase(B + 1, 0);
+    std::vector<uint64_t> ebase(B, 0), cbase(B + 1, 0);
+    uint64_t nent = 0;
+    for (size_t j = 0; j < B; ++j) {
+        const kc::HostCSR &c = gs[idx[j]]->csr;
+        vbase[j + 1] = vbase[j] + c.n;
+        ebase[j] = nent;
+        nent += c.col.size();
+        cbase[j + 1] = cbase[j] + c.col.size() / 2;
+    }
+    const uint32_t nrows = vbase[B];
+    const uint64_t rp_words = (uint64_t)nrows + B, mtot = cbase[B];
+    // pinned staging: full rowptrs, full cols, vbase, ebase
+    const size_t stage_bytes = 4 * (rp_words + nent) + 4 * (B + 1) + 8 * B + 64;
+    uint8_t *stage = nullptr;
+    if (hipHostMalloc((void **)&stage, stage_bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(KC_ERR_NOMEM, "hipHostMalloc(%zu) failed", stage_bytes);
+    uint32_t *h_frow = reinterpret_cast<uint32_t *>(stage);
+    uint32_t *h_fcol = h_frow + rp_words;
+    uint64_t *h_ebase = reinterpret_cast<uint64_t *>(stage + ((4 * (rp_words + nent) + 7) & ~(size_t)7));
+    uint32_t *h_vbase = reinterpret_cast<uint32_t *>(h_ebase + B);
+    {
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (size_t j = t; j < B; j += nt) {
+                    const kc::HostCSR &c = gs[idx[j]]->csr;
+                    std::memcpy(h_frow + vbase[j] + j, c.rowptr.data(), 4 * ((size_t)c.n + 1));
+                    if (!c.col.empty()) std::memcpy(h_fcol + ebase[j], c.col.data(), 4 * c.col.size());
+                }
+            });
+        for (auto &x : th) x.join();
+    }
+    std::memcpy(h_ebase, ebase.data(), 8 * B);
+    std::memcpy(h_vbase, vbase.data(), 4 * (B + 1));
+
+    // device: outputs in one shared arena, inputs and temporaries freed at the end
+    auto arena = std::make_shared<DevBuf>();
+    const uint64_t a_rows = rp_words, a_col = a_rows, a_src = a_col + mtot, a_key = a_src + mtot,
+                   a_words = a_key + nrows + 4;
+    DevBuf din, dtmp, dscan, dmeta;
+    int rc;
+    const size_t np = ((size_t)nrows + 2047) / 2048;
+    if ((rc = arena->ensure(4 * a_words)) || (rc = din.ensure(4 * (rp_words + nent) + 8)) ||
+        (rc = dscan.ensure(4 * (2 * (size_t)nrows + 2))) || (rc = dtmp.ensure(8 * (np + 1))) ||
+        (rc = dmeta.ensure(8 * B + 4 * (B + 1) + 4 * B + 64))) {
+        hipHostFree(stage);
+        return rc;
+    }
+    uint32_t *d_orow = arena->as<uint32_t>(), *d_ocol = d_orow + a_col, *d_osrc = d_orow + a_src,
+             *d_key = d_orow + a_key;
+    uint32_t *d_frow = din.as<uint32_t>(), *d_fcol = d_frow + rp_words;
+    uint32_t *d_outdeg = dscan.as<uint32_t>(), *d_scan = d_outdeg + nrows + 1;
+    uint64_t *d_ebase = dmeta.as<uint64_t>();
+    uint32_t *d_vbase = reinterpret_cast<uint32_t *>(d_ebase + B), *d_maxout = d_vbase + B + 1;
+    KC_HIP(hipMemcpyAsync(d_frow, h_frow, 4 * (rp_words + nent), hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemcpyAsync(d_ebase, h_ebase, 8 * B, hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemcpyAsync(d_vbase, h_vbase, 4 * (B + 1), hipMemcpyHostToDevice, s));
+
+    MultiDesc md{d_vbase, d_ebase, (uint32_t)B, nrows};
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(((uint64_t)nrows + 3) / 4, (uint64_t)ctx->cu * 16));
+    const int gmax = std::max(1, std::min(grid, 1024));
+    auto enqueue = [&]() {
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::max(1, std::min((int)((B + 255) / 256), 1024))), dim3(256), 0, s,
+                           d_maxout, (uint32_t)B, 0u);
+        hipLaunchKernelGGL(k_orient_count_multi, dim3(grid), dim3(256), 0, s, md, d_frow, d_fcol, d_key, d_outdeg);
+        const int r = scan_exclusive_u32(d_outdeg, d_scan, nrows, dtmp, s);
+        hipLaunchKernelGGL(k_orient_write_multi, dim3(grid), dim3(256), 0, s, md, d_frow, d_fcol, d_scan, d_orow,
+                           d_ocol, d_osrc);
+        hipLaunchKernelGGL(k_max_multi, dim3(gmax), dim3(256), 0, s, md, d_outdeg, d_maxout);
+        return r;
+    };
+    // one captured graph; if this runtime refuses the capture, the same launches go on the stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    bool captured = false;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int r = enqueue();
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        captured = (r == KC_OK && ce == hipSuccess && graph &&
+                    hipGraphInstantiateWithFlags(&exec, graph, 0) == hipSuccess);
+    }
+    (void)hipGetLastError();
+    hipEventRecord(e0, s);
+    hipError_t le = hipSuccess;
+    if (captured) le = hipGraphLaunch(exec, s);
+    else if ((rc = enqueue())) le = hipErrorUnknown;
+    hipEventRecord(e1, s);
+    std::vector<uint32_t> maxout(B);
+    if (le == hipSuccess) le = hipMemcpyAsync(maxout.data(), d_maxout, 4 * B, hipMemcpyDeviceToHost, s);
+    const hipError_t se = hipStreamSynchronize(s);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    hipHostFree(stage);
+    if (rc) return rc;
+    if (le != hipSuccess || se != hipSuccess)
+        return fail(KC_ERR_DEVICE, "batched orientation failed: %s", hipGetErrorString(le != hipSuccess ? le : se));
+    for (size_t j = 0; j < B; ++j) {
+        const kc::HostCSR &c = gs[idx[j]]->csr;
+        kc_oriented *o = new kc_oriented;
+        o->dev = ctx->dev;
+        o->stream = s;
+        o->n = c.n;
+        o->m = cbase[j + 1] - cbase[j];
+        o->mode = KC_ORIENT_DEGREE;
+        o->max_out = maxout[j];
+        o->max_core = 0;
+        o->pre_ms = ms;  // the whole batch's device time
+        o->rowptr.view(d_orow + vbase[j] + j, 4 * ((size_t)c.n + 1));
+        o->col.view(d_ocol + cbase[j], 4 * o->m);
+        o->src.view(d_osrc + cbase[j], 4 * o->m);
+        o->key.view(d_key + vbase[j], 4 * (size_t)c.n);
+        o->arena = arena;
+        out[idx[j]] = o;
+    }
+    return KC_OK;
+}
+
+extern "C" int kc_orient_batch(const kc_graph *const *gs, const int *modes, size_t nb, kc_oriented **out)
+{
+    if (nb && (!gs || !modes || !out)) return fail(KC_ERR_ARG, "kc_orient_batch: null argument");
+    for (size_t i = 0; i < nb; ++i) {
+        out[i] = nullptr;
+        if (!gs[i]) return fail(KC_ERR_ARG, "kc_orient_batch: null graph %zu", i);
+        if (modes[i] != KC_ORIENT_DEGREE && modes[i] != KC_ORIENT_DEGEN)
+            return fail(KC_ERR_ARG, "kc_orient_batch: bad mode %d for graph %zu", modes[i], i);
+    }
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    auto cleanup = [&](int code) {
+        for (size_t i = 0; i < nb; ++i) {
+            delete out[i];
+            out[i] = nullptr;
+        }
+        return code;
+    };
+    // chunks of at most 2^31 rows + entries each (32-bit offsets inside a chunk)
+    std::vector<size_t> chunk;
+    uint64_t words = 0;
+    for (size_t i = 0; i <= nb; ++i) {
+        const bool end = (i == nb);
+        uint64_t w = 0;
+        if (!end) {
+            const kc::HostCSR &c = gs[i]->csr;
+            if (modes[i] == KC_ORIENT_DEGEN || c.n == 0) {
+                if ((rc = kc_orient(gs[i], modes[i], &out[i]))) return cleanup(rc);
+                continue;
+            }
+            w = (uint64_t)c.n + 1 + c.col.size();
+        }
+        if (!chunk.empty() && (end || words + w >= (1ull << 31))) {
+            if ((rc = orient_degree_chunk(ctx, gs, chunk, out))) return cleanup(rc);
+            chunk.clear();
+            words = 0;
+        }
+        if (!end) {
+            chunk.push_back(i);
+            words += w;
+        }
+    }
     return KC_OK;
 }
 

@@ -412,3 +412,29 @@ def test_local_counts_cli(tmp_path, golden):
     rows = re.findall(r"^(\d+)\t(\d+)$", r.stdout, flags=re.M)
     assert [int(c) for _, c in rows] == g["local"]["5"][:50]
     assert f"kcliques={g['counts'][5]} k=5 overflow=0" in r.stdout
+
+
+def test_orient_batch_equals_one_by_one(golden):
+    """kc_orient_batch: degree graphs through one captured hipGraph over the
+    concatenation, degen and empty graphs one by one; every handle equals
+    what kc_orient gives, and counts on it are unchanged."""
+    rng = np.random.default_rng(21)
+    graphs = [_graph(g) for g in golden] + _small_mix(rng, 30)
+    graphs.append(K.Graph.from_edges(np.zeros(0, np.uint32), np.zeros(0, np.uint32), 5))  # no edges
+    graphs.append(K.Graph.from_edges(np.zeros(0, np.uint32), np.zeros(0, np.uint32), 0))  # no vertices
+    modes = ["degen" if i % 3 == 0 else "degree" for i in range(len(graphs))]
+    batch = K.orient_batch(graphs, modes)
+    for i, (G, mode, B) in enumerate(zip(graphs, modes, batch)):
+        A = G.orient(mode)
+        for x, y in zip(A.export(), B.export()):
+            assert np.array_equal(x, y), (i, mode)
+        ia, ib = A.info(), B.info()
+        assert (ia["n"], ia["m"], ia["max_out_degree"], ia["max_core"]) == (ib["n"], ib["m"], ib["max_out_degree"],
+                                                                            ib["max_core"]), (i, mode)
+        if ia["m"]:
+            assert B.count(5, "pivot", "edge").count == A.count(5, "pivot", "edge").count
+    # handles of one shared allocation stay valid when others are freed
+    keep = batch[1]
+    exp = graphs[1].orient(modes[1]).count(4, "orient", "node").count
+    del batch
+    assert keep.count(4, "orient", "node").count == exp
