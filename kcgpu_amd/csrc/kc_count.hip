@@ -352,6 +352,14 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
     return cnt;
 }
 
+// per-lane gather: lane l gets v of lane src(l)
+__device__ inline uint64_t bperm64(uint64_t v, uint32_t src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline uint64_t rdlane64(uint64_t v, uint32_t src)
 {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
@@ -402,16 +410,41 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
                         add_checked(acc, pp * cx, ovf);
                         add_checked(acc, e2 / 2, ovf);
                     } else {
-                        // triangles i < j < l of G[X], counted at i
+                        // triangles of G[X].  Deep in the dense core X is nearly a
+                        // clique, so count the complement instead when its rows are
+                        // short (Goodman: t(G) + t(~G) = C(|X|,3) - 1/2 sum_x d_x(|X|-1-d_x)):
+                        // each lane walks its complement neighbours above it and
+                        // fetches their rows with ds_bpermute.  Otherwise every
+                        // triangle i < j < l is counted at i over a uniform walk of X.
+                        const uint64_t cmpl = mem ? (X & ~row & lane_above) : 0ull;
+                        const uint32_t cmax =
+                            (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32((uint32_t)__popcll(cmpl)));
                         uint32_t tri = 0;
-                        const uint64_t mine = X & row & lane_above;
-                        for (uint64_t bits = X; bits; bits &= bits - 1) {
-                            const uint32_t il = (uint32_t)__builtin_ctzll(bits);
-                            const uint64_t above = il == 63 ? 0ull : (~0ull << (il + 1));
-                            const uint64_t Y = X & rdlane64(row, il) & above;
-                            if ((Y >> lane) & 1ull) tri += (uint32_t)__popcll(Y & mine);
+                        if (3 * cmax < 2 * cx) {
+                            uint32_t gd = mem ? v * (cx - 1 - v) : 0u, tb = 0;
+                            uint64_t rest = cmpl;
+                            for (uint32_t it = 0; it < cmax; ++it) {
+                                const uint32_t y = rest ? (uint32_t)__builtin_ctzll(rest) : 0u;
+                                const uint64_t ry = bperm64(row, y);
+                                if (rest) {
+                                    const uint64_t ay = y == 63 ? 0ull : (~0ull << (y + 1));
+                                    tb += (uint32_t)__popcll(cmpl & ~ry & ay);
+                                    rest &= rest - 1;
+                                }
+                            }
+                            gd = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(gd));
+                            tb = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tb));
+                            tri = cx * (cx - 1) * (cx - 2) / 6 - gd / 2 - tb;
+                        } else {
+                            const uint64_t mine = X & row & lane_above;
+                            for (uint64_t bits = X; bits; bits &= bits - 1) {
+                                const uint32_t il = (uint32_t)__builtin_ctzll(bits);
+                                const uint64_t above = il == 63 ? 0ull : (~0ull << (il + 1));
+                                const uint64_t Y = X & rdlane64(row, il) & above;
+                                if ((Y >> lane) & 1ull) tri += (uint32_t)__popcll(Y & mine);
+                            }
+                            tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
                         }
-                        tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) * (pp >= 2 ? pp - 2 : 0) / 6, ovf);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2 * cx, ovf);
                         add_checked(acc, pp * (e2 / 2), ovf);
