@@ -637,9 +637,9 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                         uint64_t x = X[w];
                         Cc[w] = x;
                         B[w] = x & ~rp[w];
-                        if (LOCAL) {
-                            B[2 * wmax + w] = Hc[w];
-                            B[3 * wmax + w] = Pc[w];
+                        if (LOCAL) {  // level layout: Cc, B, held, pivots (wmax words each)
+                            Cc[2 * wmax + w] = Hc[w];
+                            Cc[3 * wmax + w] = Pc[w];
                         }
                     }
                     if (lane == 0) scal[t] = Scal{h, p, pv, 0};
@@ -669,8 +669,8 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
             X[w] = c & rb[w];
             if (LOCAL) {
                 const uint64_t bit = (w == (b >> 6)) ? (1ull << (b & 63)) : 0ull;
-                Hc[w] = B[2 * wmax + w] | (b != sc.pivot ? bit : 0ull);
-                Pc[w] = B[3 * wmax + w] | (b == sc.pivot ? bit : 0ull);
+                Hc[w] = Cc[2 * wmax + w] | (b != sc.pivot ? bit : 0ull);
+                Pc[w] = Cc[3 * wmax + w] | (b == sc.pivot ? bit : 0ull);
             }
             if (w == (b >> 6)) {
                 Cc[w] = c & ~(1ull << (b & 63));
