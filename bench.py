@@ -29,6 +29,7 @@ METRIC = "k-cliques/s and edges/s at k=5..8; serving p50/p99 vs per-call plain k
 HBM_PEAK_GBS = 8000.0
 CL = dict(n=1_700_000, m=11_100_000, gamma=2.1, max_degree=35000.0)
 SEED = 2
+ORIENT_BOUND = 2e9  # serving: largest clique-enumeration bound an orientation-mode query keeps
 
 
 def log(*a):
@@ -172,6 +173,21 @@ def run_serve(args):
         graphs = list(ex.map(lambda sp: make_graph(K, sp), specs))
     plans = [G.autotune(8) for G in graphs]
     log(f"generated {len(graphs)} graphs in {time.perf_counter() - t:.1f}s")
+    # Orientation-mode queries enumerate every clique; the client sends one as
+    # pivoting (same count) when the enumeration bound sum_u C(d+(u), k-1) of
+    # its graph is above ORIENT_BOUND.  Out-degrees come from one orientation.
+    from math import comb
+    outdeg_hist = []
+    for G, pl in zip(graphs, plans):
+        o = G.orient(pl["orient"])
+        rp, _, _ = o.export()
+        d = np.diff(rp.astype(np.int64))
+        outdeg_hist.append(np.bincount(d))
+        del o
+
+    def orient_bound(gi, k):
+        h = outdeg_hist[gi]
+        return sum(int(c) * comb(int(dv), k - 1) for dv, c in enumerate(h) if c and dv >= k - 1)
     srv = K.Server(workers=args.workers, max_batch=args.max_batch)
     gids = [None] * len(graphs)
     t = time.perf_counter()
@@ -186,6 +202,7 @@ def run_serve(args):
     modes = rng.integers(0, 2, nq)
     tickets, sent = [None] * nq, [None] * nq
     live = args.resident
+    converted = 0
 
     def uploader():  # 64 graphs arrive during the run
         nonlocal live
@@ -205,10 +222,19 @@ def run_serve(args):
         if d > 0:
             time.sleep(d)
         gi = int(pick[i] * live)
-        q = dict(k=int(ks[i]), algo="pivot" if modes[i] else "orient", gran=plans[gi]["gran"], part_size=1)
+        algo = "pivot" if modes[i] else "orient"
+        if algo == "orient" and orient_bound(gi, int(ks[i])) > ORIENT_BOUND:
+            algo = "pivot"
+            converted += 1
+        q = dict(k=int(ks[i]), algo=algo, gran=plans[gi]["gran"], part_size=1)
         sent[i] = (gi, q)
         tickets[i] = srv.submit(gids[gi], **q)
+        if (i + 1) % 100 == 0:
+            log(f"submitted {i + 1}/{nq}", srv.stats())
     up.join()
+    while srv.stats()["completed"] < nq:  # drain, printing progress
+        time.sleep(2.0)
+        log("draining", srv.stats())
     srv.drain()
     wall = time.perf_counter() - t0
     st = srv.stats()
@@ -218,10 +244,12 @@ def run_serve(args):
     ors = {}
     t = time.perf_counter()
     plain = []
-    for gi, q in sent:
+    for i, (gi, q) in enumerate(sent):
         if gi not in ors:
             ors[gi] = graphs[gi].orient(plans[gi]["orient"])
         plain.append(ors[gi].count(**q).count)
+        if (i + 1) % 100 == 0:
+            log(f"plain {i + 1}/{nq}")
     t_plain = time.perf_counter() - t
     same = got == plain
     log(f"plain back-to-back {t_plain:.2f}s; identical={same}")
@@ -235,6 +263,7 @@ def run_serve(args):
                                   f"R-MAT, BA; <=2M edges); {nq} Poisson queries at {args.rate}/s, k~U{{3..10}}, "
                                   "orient|pivot, arrival seed 55",
                       "offered_rate": args.rate, "wall_s": wall, "batches": st["batches"],
+                      "orient_queries_sent_as_pivot": converted, "orient_bound": ORIENT_BOUND,
                       "mean_batch": st["mean_batch"], "workers": args.workers,
                       "plain_back_to_back_s": t_plain, "plain_queries_per_s": nq / t_plain,
                       "resident_bytes": st["resident_bytes"]}}
