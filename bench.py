@@ -244,10 +244,13 @@ def run_serve(args):
     ors = {}
     t = time.perf_counter()
     plain = []
+    each = []
     for i, (gi, q) in enumerate(sent):
         if gi not in ors:
             ors[gi] = graphs[gi].orient(plans[gi]["orient"])
+        t1 = time.perf_counter()
         plain.append(ors[gi].count(**q).count)
+        each.append((time.perf_counter() - t1) * 1e3)
         if (i + 1) % 100 == 0:
             log(f"plain {i + 1}/{nq}")
     t_plain = time.perf_counter() - t
@@ -266,6 +269,10 @@ def run_serve(args):
                       "orient_queries_sent_as_pivot": converted, "orient_bound": ORIENT_BOUND,
                       "mean_batch": st["mean_batch"], "workers": args.workers,
                       "plain_back_to_back_s": t_plain, "plain_queries_per_s": nq / t_plain,
+                      "plain_ms_p50": float(np.percentile(each, 50)), "plain_ms_p99": float(np.percentile(each, 99)),
+                      "plain_ms_max": float(max(each)),
+                      "heaviest": [dict(kind=specs[sent[j][0]]["kind"], k=sent[j][1]["k"], algo=sent[j][1]["algo"],
+                                        ms=round(each[j], 1)) for j in np.argsort(each)[-5:][::-1]],
                       "resident_bytes": st["resident_bytes"]}}
     print(json.dumps(out), flush=True)
 
@@ -351,9 +358,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=4096)
     ap.add_argument("--resident", type=int, default=512)
     ap.add_argument("--uploads", type=int, default=64)
-    ap.add_argument("--queries", type=int, default=2000)
-    ap.add_argument("--rate", type=float, default=200.0)
-    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--queries", type=int, default=600)
+    ap.add_argument("--rate", type=float, default=8.0)
+    ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

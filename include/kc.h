@@ -292,7 +292,8 @@ typedef struct kc_server kc_server;
 
 typedef struct {
     uint64_t budget_bytes;  /* HBM for resident CSRs + launch scratch; 0 = 90% of free memory */
-    uint32_t workers;       /* batches in flight at once (one stream each); 0 = 2 */
+    uint32_t workers;       /* batches in flight at once (one stream and 1/workers of the
+                               device's block slots each); 0 = 4 */
     uint32_t max_batch;     /* queries per batch; 0 = 256 */
 } kc_server_config;
 

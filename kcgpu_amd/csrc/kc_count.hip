@@ -1276,7 +1276,7 @@ static void binom_table(uint32_t nmax, uint32_t rmax, std::vector<uint64_t> &val
 }
 
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
-                LaunchStats *ls, uint64_t *local)
+                LaunchStats *ls, uint64_t *local, uint32_t grid_share)
 {
     DeviceCtx *ctx;
     int rc = device_ctx(&ctx);
@@ -1428,7 +1428,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         int per_cu = 0;
         KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
         if (per_cu < 1) per_cu = 1;
-        uint64_t blocks = (uint64_t)per_cu * ctx->cu;
+        uint64_t blocks = std::max<uint64_t>(1, (uint64_t)per_cu * ctx->cu / std::max<uint32_t>(1, grid_share));
         a.chunk = (c == 0 && bcnt[b] > blocks * 64) ? 4 : 1;
         blocks = std::min<uint64_t>(blocks, ((uint64_t)bcnt[b] + a.chunk - 1) / a.chunk);
         if (global) {

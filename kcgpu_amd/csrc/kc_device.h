@@ -76,8 +76,11 @@ struct LaunchStats {
 // the current device): one launch per (algorithm, size class) bin.  With
 // local != null (one query only) the per-vertex counts are added into
 // local[0..n) on the device as well.
+// grid_share > 1 caps every persistent launch at 1/grid_share of the
+// device's resident blocks, so that launches of that many streams run side
+// by side (the server's workers) instead of the first one filling the GPU.
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
-                LaunchStats *ls, uint64_t *local = nullptr);
+                LaunchStats *ls, uint64_t *local = nullptr, uint32_t grid_share = 1);
 
 // query validation and the autotuner's choice of algo / gran / P for a query
 int check_query(const kc_query *q);

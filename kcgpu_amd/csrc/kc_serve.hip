@@ -13,7 +13,10 @@
 //     persistent launches draining a queue of (query, task).  Queries that
 //     arrive while a batch is on the GPU go into the next batch, which the
 //     other worker launches on its stream, so consecutive batches overlap on
-//     the device.
+//     the device.  Each worker's persistent launches take 1/workers of the
+//     device's resident block slots, so a long batch (a dense graph at a large
+//     k) leaves room for the short batches behind it instead of filling every
+//     CU until it ends.
 //   * Latency is measured from submit to the moment the worker has the result.
 #include <hip/hip_runtime.h>
 
@@ -136,7 +139,7 @@ void worker_main(kc_server *s)
             for (const Ticket &t : batch) runs.push_back(kc::QueryRun{s->graphs[t.gid].o, t.q});
         }
         std::vector<kc_result> res;
-        int rc = kc::run_queries(scr, st, runs, res, nullptr);
+        int rc = kc::run_queries(scr, st, runs, res, nullptr, nullptr, s->cfg.workers);
         const auto t_end = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -188,7 +191,7 @@ int kc_server_create(const kc_server_config *cfg, kc_server **out)
     if (rc) return rc;
     kc_server *s = new kc_server;
     if (cfg) s->cfg = *cfg;
-    if (s->cfg.workers == 0) s->cfg.workers = 2;
+    if (s->cfg.workers == 0) s->cfg.workers = 4;
     if (s->cfg.workers > 8) s->cfg.workers = 8;
     if (s->cfg.max_batch == 0) s->cfg.max_batch = 256;
     if (s->cfg.budget_bytes == 0) {

@@ -53,10 +53,10 @@ def test_orientation_matches_oracle(golden, mode):
         assert Og.info()["max_out_degree"] == int(np.diff(erp).max(initial=0))
 
 
-def test_stepped_peel_matches_oracle(golden, monkeypatch):
-    """The peel without a cooperative launch (taken when the device's
-    cooperative resource is held by another process) gives the same key."""
-    monkeypatch.setenv("KC_PEEL_MODE", "host")
+def test_cooperative_peel_matches_oracle(golden, monkeypatch):
+    """The single cooperative peel kernel (KC_PEEL_MODE=coop; the default is
+    the stepped peel, covered by every other degen test) gives the same key."""
+    monkeypatch.setenv("KC_PEEL_MODE", "coop")
     graphs = [_graph(g) for g in golden]
     graphs.append(K.Graph.generate(K.GEN_CHUNG_LU, seed=3, n=20000, m=150000, gamma=2.1, max_degree=2000))
     for G in graphs:
