@@ -317,6 +317,70 @@ def run_config(args):
         sys.exit(1)
 
 
+def run_lkc(args):
+    """-m lkc on the headline graph: per-vertex counts at each k (--ks),
+    checked by sum_v local[v] = k * count against the plain count."""
+    import numpy as np
+    import kcgpu_amd as K
+    G = K.Graph.generate(K.GEN_CHUNG_LU, seed=SEED, **CL)
+    Og = G.orient("degen")
+    rows = []
+    for k in [int(x) for x in args.ks.split(",")]:
+        loc, r = Og.count_local(k, "pivot", "edge", part_size=1, check=False)
+        plain = Og.count(k, "pivot", "edge", part_size=1, check=False)
+        ok = int(loc.sum(dtype=np.uint64)) == (k * plain.count) % (1 << 64) and r.count == plain.count
+        row = {"k": k, "count": r.count, "device_ms": round(r.raw["count_ms"], 3),
+               "plain_device_ms": round(plain.raw["count_ms"], 3), "sum_check": ok,
+               "max_local": int(loc.max()), "vertices_in_a_clique": int((loc > 0).sum())}
+        rows.append(row)
+        log(json.dumps(row))
+        if not ok:
+            sys.exit(1)
+    print(json.dumps({"workload": "lkc", "graph": "chung-lu n=1.7M m=11.1M seed 2, degen, edge pivoting",
+                      "per_k": rows}), flush=True)
+
+
+def run_formats(args):
+    """On-disk formats on the headline graph (host only): write it as TSV,
+    convert tsv-bel and bel-mtx, load each, and load the oriented-CSR cache."""
+    import tempfile
+    import numpy as np
+    import kcgpu_amd as K
+    G = K.Graph.generate(K.GEN_CHUNG_LU, seed=SEED, **CL)
+    rp, col = G.csr()
+    src = np.repeat(np.arange(len(rp) - 1, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    keep = src < col
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        tsv, bel, mtx, kco = (os.path.join(d, "g" + e) for e in (".tsv", ".bel", ".mtx", ".kco"))
+        w = np.empty((int(keep.sum()), 3), dtype=np.uint64)
+        w[:, 0], w[:, 1], w[:, 2] = col[keep], src[keep], 1
+        import pandas as pd
+        pd.DataFrame(w).to_csv(tsv, sep=" ", header=False, index=False)
+        sums = {}
+        for name, fn in (("tsv-bel", lambda: K.convert("tsv-bel", tsv, bel)),
+                         ("bel-mtx", lambda: K.convert("bel-mtx", bel, mtx)),
+                         ("load_tsv", lambda: K.Graph.load(tsv)), ("load_bel", lambda: K.Graph.load(bel)),
+                         ("load_mtx", lambda: K.Graph.load(mtx))):
+            t = time.perf_counter()
+            r = fn()
+            out[name + "_s"] = round(time.perf_counter() - t, 3)
+            if r is not None:
+                sums[name] = r.info()["checksum"]
+        out["same_graph_from_every_format"] = len(set(sums.values()) | {G.info()["checksum"]}) == 1
+        out["tsv_bytes"], out["bel_bytes"], out["mtx_bytes"] = (os.path.getsize(x) for x in (tsv, bel, mtx))
+        t = time.perf_counter()
+        Og = K.Graph.load(bel).orient("degen")
+        out["load_bel_and_orient_s"] = round(time.perf_counter() - t, 3)
+        Og.save(kco)
+        t = time.perf_counter()
+        L = K.Oriented.load(kco)
+        out["load_kco_s"] = round(time.perf_counter() - t, 3)
+        out["kco_bytes"] = os.path.getsize(kco)
+        out["kco_count_k5_equal"] = L.count(5, "pivot", "edge").count == Og.count(5, "pivot", "edge").count
+    print(json.dumps({"workload": "formats", "graph": "chung-lu n=1.7M m=11.1M seed 2", **out}), flush=True)
+
+
 def run_copy(args):
     """Counter calibration: device-to-device copies of a known size through the
     HIP runtime's copy kernel, so rocprofv3's FETCH_SIZE / WRITE_SIZE can be
@@ -348,11 +412,12 @@ def main():
     ap.add_argument("--ks", default="5,6,7,8")
     ap.add_argument("--part-size", type=int, default=0, help="0 = autotune")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=2000)
+    ap.add_argument("--cpu-stride", type=int, default=30)
     ap.add_argument("--copy-mib", type=int, default=1024)
-    ap.add_argument("--workload", choices=["c2", "batch", "serve", "copy", "c1", "c3"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batch", "serve", "copy", "c1", "c3", "lkc", "formats"], default="c2",
                     help="c2: the headline k=5..8 count (default); batch: the 4096-graph batched mix; "
-                         "serve: the serving loop; c1 / c3: the ER and R-MAT configurations")
+                         "serve: the serving loop; c1 / c3: the ER and R-MAT configurations; "
+                         "lkc: per-vertex counts; formats: converters and the oriented-CSR cache")
     ap.add_argument("--combos", default="degen:edge,degen:node,degree:edge,degree:node")
     ap.add_argument("--c3-ks", default="3,4,5,6,7,8,9,10,11")
     ap.add_argument("--graphs", type=int, default=4096)
@@ -377,6 +442,10 @@ def main():
         return run_copy(args)
     if args.workload in ("c1", "c3"):
         return run_config(args)
+    if args.workload == "lkc":
+        return run_lkc(args)
+    if args.workload == "formats":
+        return run_formats(args)
 
     import numpy as np
     import kcgpu_amd as K
