@@ -367,6 +367,104 @@ __device__ inline uint64_t rdlane64(uint64_t v, uint32_t src)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// ---- closed forms over a one-word candidate set X (row = the lane's row) ----
+// All of them run with every lane active: rows of other lanes come in with
+// ds_bpermute, and loops run while any lane still has work (ballot).
+
+// triangles of G[X] through the complement (Goodman):
+//   t(G) = C(|X|,3) - 1/2 sum_x d_x (|X|-1-d_x) - t(complement),
+// each lane walking its complement neighbours above it (cmpl, at most cmax)
+__device__ inline uint32_t tri_complement(uint64_t row, uint64_t cmpl, uint32_t cmax, bool mem, uint32_t v, uint32_t cx)
+{
+    uint32_t gd = mem ? v * (cx - 1 - v) : 0u, tb = 0;
+    uint64_t rest = cmpl;
+    for (uint32_t it = 0; it < cmax; ++it) {
+        const uint32_t y = rest ? (uint32_t)__builtin_ctzll(rest) : 0u;
+        const uint64_t ry = bperm64(row, y);
+        if (rest) {
+            const uint64_t ay = y == 63 ? 0ull : (~0ull << (y + 1));
+            tb += (uint32_t)__popcll(cmpl & ~ry & ay);
+            rest &= rest - 1;
+        }
+    }
+    gd = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(gd));
+    tb = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tb));
+    return cx * (cx - 1) * (cx - 2) / 6 - gd / 2 - tb;
+}
+
+// triangles of G[X], each i < j < l counted at i over a uniform walk of X
+__device__ inline uint32_t tri_direct(uint64_t X, uint64_t row, uint64_t lane_above)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    uint32_t tri = 0;
+    const uint64_t mine = X & row & lane_above;
+    for (uint64_t bits = X; bits; bits &= bits - 1) {
+        const uint32_t il = (uint32_t)__builtin_ctzll(bits);
+        const uint64_t above = il == 63 ? 0ull : (~0ull << (il + 1));
+        const uint64_t Y = X & rdlane64(row, il) & above;
+        if ((Y >> lane) & 1ull) tri += (uint32_t)__popcll(Y & mine);
+    }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
+}
+
+// 4-cliques of G[X], counted at their smallest vertex x: lane x counts the
+// triangles of its forward neighbourhood Y = X & N(x) & (> x) by Goodman's
+// identity inside Y, walking y in Y (degree of y in Y) and, for the
+// complement triangles of Y, the complement neighbours z of y inside Y.
+__device__ inline uint32_t k4_forward(uint64_t X, uint64_t row, bool mem, uint64_t lane_above)
+{
+    const uint64_t Y = mem ? (X & row & lane_above) : 0ull;
+    const uint32_t ny = (uint32_t)__popcll(Y);
+    uint32_t sq = 0, tb = 0;
+    uint64_t rest = Y;
+    while (__ballot(rest != 0)) {
+        const uint32_t y = rest ? (uint32_t)__builtin_ctzll(rest) : 0u;
+        const uint64_t ry = bperm64(row, y);
+        uint64_t cy = 0;
+        if (rest) {
+            const uint32_t dy = (uint32_t)__popcll(Y & ry);
+            sq += dy * (ny - 1 - dy);
+            cy = Y & ~ry & (y == 63 ? 0ull : (~0ull << (y + 1)));
+            rest &= rest - 1;
+        }
+        uint64_t r2 = cy;
+        while (__ballot(r2 != 0)) {
+            const uint32_t z = r2 ? (uint32_t)__builtin_ctzll(r2) : 0u;
+            const uint64_t rz = bperm64(row, z);
+            if (r2) {
+                tb += (uint32_t)__popcll(cy & ~rz & (z == 63 ? 0ull : (~0ull << (z + 1))));
+                r2 &= r2 - 1;
+            }
+        }
+    }
+    const uint32_t mine = ny >= 3 ? ny * (ny - 1) * (ny - 2) / 6 - sq / 2 - tb : 0u;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(mine));
+}
+
+// X is a near-clique: no member has more complement neighbours above it than
+// two thirds of |X| (the complement walks stay short)
+__device__ inline bool near_clique(uint64_t X, uint64_t row, bool mem, uint64_t lane_above, uint32_t cx)
+{
+    const uint32_t c = mem ? (uint32_t)__popcll(X & ~row & lane_above) : 0u;
+    const uint32_t cmax = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32(c));
+    return 3 * cmax < 2 * cx;
+}
+
+__device__ inline uint64_t mul_chk(uint64_t x, uint64_t y, uint32_t &ovf)
+{
+    uint64_t r;
+    if (__builtin_mul_overflow(x, y, &r)) ovf = 1;
+    return r;
+}
+
+__device__ inline uint64_t binom_chk(const CountArgs &a, uint32_t n, uint32_t r, uint32_t &ovf)
+{
+    if (r > n) return 0;
+    const uint64_t idx = (uint64_t)n * a.bstride + r;
+    if (a.binom_ovf[idx]) ovf = 1;
+    return a.binom[idx];
+}
+
 // SCT of a subproblem of at most 64 vertices held one row per lane: lane i
 // keeps row i (bit j <=> vertex j of the subproblem is adjacent), X0 is the
 // candidate set, h0/p0 the held and pivot counts above it.  Every set is one
@@ -410,46 +508,32 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
                         add_checked(acc, pp * cx, ovf);
                         add_checked(acc, e2 / 2, ovf);
                     } else {
-                        // triangles of G[X].  Deep in the dense core X is nearly a
-                        // clique, so count the complement instead when its rows are
-                        // short (Goodman: t(G) + t(~G) = C(|X|,3) - 1/2 sum_x d_x(|X|-1-d_x)):
-                        // each lane walks its complement neighbours above it and
-                        // fetches their rows with ds_bpermute.  Otherwise every
-                        // triangle i < j < l is counted at i over a uniform walk of X.
+                        // triangles of G[X]: through the complement when X is nearly a
+                        // clique, else every triangle i < j < l counted at i
                         const uint64_t cmpl = mem ? (X & ~row & lane_above) : 0ull;
                         const uint32_t cmax =
                             (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32((uint32_t)__popcll(cmpl)));
-                        uint32_t tri = 0;
-                        if (3 * cmax < 2 * cx) {
-                            uint32_t gd = mem ? v * (cx - 1 - v) : 0u, tb = 0;
-                            uint64_t rest = cmpl;
-                            for (uint32_t it = 0; it < cmax; ++it) {
-                                const uint32_t y = rest ? (uint32_t)__builtin_ctzll(rest) : 0u;
-                                const uint64_t ry = bperm64(row, y);
-                                if (rest) {
-                                    const uint64_t ay = y == 63 ? 0ull : (~0ull << (y + 1));
-                                    tb += (uint32_t)__popcll(cmpl & ~ry & ay);
-                                    rest &= rest - 1;
-                                }
-                            }
-                            gd = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(gd));
-                            tb = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tb));
-                            tri = cx * (cx - 1) * (cx - 2) / 6 - gd / 2 - tb;
-                        } else {
-                            const uint64_t mine = X & row & lane_above;
-                            for (uint64_t bits = X; bits; bits &= bits - 1) {
-                                const uint32_t il = (uint32_t)__builtin_ctzll(bits);
-                                const uint64_t above = il == 63 ? 0ull : (~0ull << (il + 1));
-                                const uint64_t Y = X & rdlane64(row, il) & above;
-                                if ((Y >> lane) & 1ull) tri += (uint32_t)__popcll(Y & mine);
-                            }
-                            tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
-                        }
+                        const uint32_t tri = (3 * cmax < 2 * cx) ? tri_complement(row, cmpl, cmax, mem, v, cx)
+                                                                 : tri_direct(X, row, lane_above);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) * (pp >= 2 ? pp - 2 : 0) / 6, ovf);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2 * cx, ovf);
                         add_checked(acc, pp * (e2 / 2), ovf);
                         add_checked(acc, tri, ovf);
                     }
+                } else if (h + 4 == k && near_clique(X, row, mem, lane_above, cx)) {
+                    // h = k-4 over a near-clique X: the subtree holds
+                    // C(p,4) + C(p,3)|X| + C(p,2)|E(X)| + p c3(X) + c4(X) k-cliques
+                    const uint64_t cmpl = mem ? (X & ~row & lane_above) : 0ull;
+                    const uint32_t cmax =
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32((uint32_t)__popcll(cmpl)));
+                    const uint64_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(v)) / 2;
+                    const uint64_t c3 = tri_complement(row, cmpl, cmax, mem, v, cx);
+                    const uint64_t c4 = k4_forward(X, row, mem, lane_above);
+                    add_checked(acc, binom_chk(a, p, 4, ovf), ovf);
+                    add_checked(acc, mul_chk(binom_chk(a, p, 3, ovf), cx, ovf), ovf);
+                    add_checked(acc, mul_chk(binom_chk(a, p, 2, ovf), e, ovf), ovf);
+                    add_checked(acc, mul_chk(p, c3, ovf), ovf);
+                    add_checked(acc, c4, ovf);
                 } else {
                     // (score + 1, 63 - index): the max names the smallest index of the largest score
                     uint32_t key = mem ? (((v + 1) << 8) | (63u - lane)) : 0u;
@@ -1016,8 +1100,10 @@ __device__ __attribute__((always_inline)) void finish_wave(const CountArgs &a, W
 }
 
 // register-bitmap pivoting kernel for tasks of at most 64*S vertices
+// waves per SIMD the register budget is held to: 3 for S = 4 (<= 168 VGPRs), 4 for S = 2, 5 for S = 1
 template <int S>
-__global__ __launch_bounds__(64) void k_pivot_reg(CountArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S == 4 ? 3 : S == 2 ? 4 : 5)))
+void k_pivot_reg(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
     const int lane = lane_id();
