@@ -521,7 +521,8 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    names = {1: "k_count<pivot,LDS>", 2: f"k_pivot_reg<{[1, 2, 4][dc] if dc < 3 else '?'}>", 3: "k_count<pivot,HBM>"}
+    names = {1: "k_count<pivot,LDS>", 2: f"k_pivot_reg<{[1, 2, 4][dc] if dc < 3 else '?'}>", 3: "k_count<pivot,HBM>",
+             4: "k_pivot_wide<LDS>", 5: "k_pivot_wide<HBM>"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": f"{names.get(cls_var[dc], '?')} (size class {dc}, tasks of <= {[64, 128, 256, 512, 1024, 'more'][dc]} vertices)",

@@ -215,7 +215,8 @@ typedef struct {
     uint64_t dominant_bytes; /* algorithmic bytes of that kernel's tasks */
     uint32_t dominant_class; /* its size class (0: <=64, 1: <=128, 2: <=256, 3: <=512, 4: <=1024, 5: larger) */
     /* per size class: device time, algorithmic bytes, tasks, kernel variant
-     * (0 none, 1 LDS bitmap, 2 register bitmap, 3 HBM bitmap) */
+     * (0 none, 1 LDS bitmap, 2 register bitmap, 3 HBM bitmap, 4 LDS bitmap
+     * shared by a block of waves, 5 HBM bitmap shared by a block of waves) */
     double class_ms[6];
     uint64_t class_bytes[6];
     uint64_t class_tasks[6];

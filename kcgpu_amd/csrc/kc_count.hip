@@ -228,14 +228,9 @@ __device__ __attribute__((always_inline)) void build_rows(const Cur &g, uint32_t
     const uint32_t total = carry;
     bytes += 8ull * R + 4ull * total;
     const uint32_t lo_v = list[0], hi_v = list[R - 1];
+    uint32_t lo = 0;  // row of entry f: the last i with offs[i] <= f (f only grows)
     for (uint32_t f = lane; f < total; f += 64) {
-        // row i: the last i with offs[i] <= f
-        uint32_t lo = 0, hi = R - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (offs[mid] <= f) lo = mid;
-            else hi = mid - 1;
-        }
+        while (offs[lo + 1] <= f) ++lo;
         uint32_t x = g.col[starts[lo] + (f - offs[lo])];
         if (x < lo_v || x > hi_v) continue;
         uint32_t j = lower_bound_u32(list, R, x);
@@ -608,10 +603,27 @@ __device__ inline uint64_t binom_at(const CountArgs &a, int64_t n, int64_t r)
     return (r < 0 || n < 0 || r > n) ? 0ull : a.binom[(uint64_t)n * a.bstride + (uint64_t)r];
 }
 
-template <bool LOCAL>
+// Barrier for the state of one traversal: the whole block when a block is
+// one wave, only the wave when the waves of a block run separate subtrees.
+template <bool WIDE>
+__device__ inline void sync_tree()
+{
+    if (WIDE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// The SCT from the root candidate set (all R task vertices, held h0, no
+// pivots), or -- with init_x false -- from the candidate set already in X
+// with h0 held and p0 pivot vertices (a subtree handed to one wave of a block).
+template <bool LOCAL, bool WIDE = false>
 __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
                                 Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
-                                uint32_t &ovf, uint64_t *lacc)
+                                uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -621,15 +633,16 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
     uint64_t *Hc = X + wmax, *Pc = X + 2 * wmax;     // LOCAL: held / pivot sets of the node
     uint64_t acc = 0;
     // root candidate set = all R list vertices
-    for (uint32_t w = lane; w < W; w += 64) {
-        uint32_t lo = w * 64;
-        X[w] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
-        if (LOCAL) Hc[w] = Pc[w] = 0;
-    }
-    __syncthreads();
+    if (init_x)
+        for (uint32_t w = lane; w < W; w += 64) {
+            uint32_t lo = w * 64;
+            X[w] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
+            if (LOCAL) Hc[w] = Pc[w] = 0;
+        }
+    sync_tree<WIDE>();
     int t = -1;
     bool pending = true;
-    uint32_t h = h0, p = 0;
+    uint32_t h = h0, p = p0;
     uint64_t my_nodes = 0;
     // LOCAL: add vh to each held and vp to each pivot vertex of the node
     auto credit_hp = [&](uint64_t vh, uint64_t vp) {
@@ -652,7 +665,7 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                 if ((xw >> lane) & 1ull) cand[cx + (uint32_t)__popcll(xw & lt)] = w * 64 + (uint32_t)lane;
                 cx += (uint32_t)__popcll(xw);
             }
-            __syncthreads();
+            sync_tree<WIDE>();
             if (cx == 0) {
                 if (k >= h && k - h <= p) {
                     uint64_t idx = (uint64_t)p * a.bstride + (k - h);
@@ -727,7 +740,7 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                         }
                     }
                     if (lane == 0) scal[t] = Scal{h, p, pv, 0};
-                    __syncthreads();
+                    sync_tree<WIDE>();
                 }
             }
         }
@@ -761,7 +774,7 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                 B[w] &= ~(1ull << (b & 63));
             }
         }
-        __syncthreads();
+        sync_tree<WIDE>();
         h = sc.held + (b != sc.pivot ? 1u : 0u);
         p = sc.piv + (b == sc.pivot ? 1u : 0u);
         if (last) --t;  // the pivot branch replaces its level
@@ -1125,6 +1138,11 @@ void k_pivot_reg(CountArgs a)
             const Cur g = select_task(a, a.tasks[ti], w, task);
             ++w.ntask;
             const uint32_t R = stage_task(g, task, list, w.bytes);
+            if (g.gran && g.k == 3) {  // an edge task at k=3 adds |T| (no bitmap needed)
+                add_checked(w.run, R, w.run_ovf);
+                __syncthreads();
+                continue;
+            }
             __syncthreads();
             build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
             add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf), w.run_ovf);
@@ -1191,7 +1209,7 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
             const uint32_t R = stage_task(g, task, list, w.bytes);
             if (LOCAL)
                 for (uint32_t i = lane; i < R; i += 64) lacc[i] = 0;
-            if (g.gran && !PIVOT && k == 3) {  // edge orientation at k=3 adds |T|
+            if (g.gran && k == 3) {  // an edge task at k=3 adds |T| (no bitmap needed)
                 add_checked(w.run, R, w.run_ovf);
                 if (LOCAL) {
                     for (uint32_t i = lane; i < R; i += 64) lacc[i] = 1;
@@ -1224,6 +1242,194 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
             __syncthreads();
             publish_local<LOCAL>(a, g, task, list, lacc, R, task_count);
             if (LOCAL) __syncthreads();
+            const uint64_t dt = wall_clock64() - t0;
+            w.tmax = dt > w.tmax ? dt : w.tmax;
+            w.tsum += dt;
+        }
+    }
+    finish_wave(a, w);
+}
+
+// ---- the wide kernel: one block of NW waves per large task -----------------
+// Tasks of 257..1024 vertices (LDS) and above (HBM slice) hold a bitmap of up
+// to 128 KB, so a one-wave block leaves most of a CU idle.  Here the NW waves
+// of a block share one task: they build its bitmap together, wave 0 picks the
+// root pivot, and the root's branches -- the pivot branch first, then each
+// non-pivot vertex b with the candidates (T minus the non-pivot branches
+// before b) & N(b) -- are handed to the waves through an LDS counter; each
+// wave runs its subtree with its own stack (pivot_count<false, true>).
+struct WideLayout {
+    uint64_t ctl, list, offs, starts, rows, wave, per_wave, total;
+};
+
+__host__ __device__ inline WideLayout make_wide_layout(uint32_t rmax, uint32_t wmax, uint32_t levels, uint32_t nw)
+{
+    WideLayout L;
+    L.ctl = 0;
+    L.list = L.ctl + 8;
+    L.offs = L.list + (rmax + 1) / 2;
+    L.starts = L.offs + (rmax + 2) / 2;
+    L.rows = L.starts + (rmax + 1) / 2 + wmax;  // starts is followed by the root's branch set
+    L.wave = L.rows + (uint64_t)rmax * wmax;
+    L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2;
+    L.total = L.wave + L.per_wave * nw;
+    return L;
+}
+
+// bitmap rows of the task by every thread of the block (build_rows with the
+// row offsets scanned by wave 0)
+__device__ void build_rows_block(const Cur &g, const uint32_t *list, uint32_t *offs, uint32_t *starts, uint64_t *rows,
+                                 uint32_t R, uint32_t W, uint64_t &bytes)
+{
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    for (uint32_t i = tid; i < R * W; i += nt) rows[i] = 0;
+    if (tid < 64) {
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < R; base += 64) {
+            const uint32_t i = base + lane;
+            uint32_t d = 0;
+            if (i < R) {
+                const uint32_t w = list[i];
+                const uint32_t st = g.rowptr[w];
+                d = g.rowptr[w + 1] - st;
+                starts[i] = st;
+            }
+            const uint32_t inc = wave_incl_scan32(d);
+            if (i < R) offs[i] = carry + inc - d;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (lane == 0) offs[R] = carry;
+        bytes += 8ull * R + 4ull * carry;
+    }
+    __syncthreads();
+    const uint32_t total = offs[R];
+    const uint32_t lo_v = list[0], hi_v = list[R - 1];
+    uint32_t lo = 0;  // row of entry f (f only grows)
+    for (uint32_t f = tid; f < total; f += nt) {
+        while (offs[lo + 1] <= f) ++lo;
+        const uint32_t x = g.col[starts[lo] + (f - offs[lo])];
+        if (x < lo_v || x > hi_v) continue;
+        const uint32_t j = lower_bound_u32(list, R, x);
+        if (j < R && list[j] == x) {
+            atomicOr((unsigned long long *)&rows[(uint64_t)lo * W + (j >> 6)], 1ull << (j & 63));
+            atomicOr((unsigned long long *)&rows[(uint64_t)j * W + (lo >> 6)], 1ull << (lo & 63));
+        }
+    }
+    __syncthreads();
+}
+
+template <bool GLOBAL>
+__global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
+{
+    extern __shared__ uint64_t lds[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, NW = blockDim.x >> 6;
+    uint64_t *ws = GLOBAL ? a.gscr + (uint64_t)blockIdx.x * a.gscr_words : lds;
+    const WideLayout L = make_wide_layout(a.rmax, a.wmax, a.levels, NW);
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(ws + L.ctl);  // task, R, next branch, branches, pivot
+    uint32_t *list = reinterpret_cast<uint32_t *>(ws + L.list);
+    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the branch list
+    uint32_t *starts = reinterpret_cast<uint32_t *>(ws + L.starts);
+    uint64_t *bnp = ws + L.rows - a.wmax;                        // non-pivot branch set of the root
+    uint64_t *rows = ws + L.rows;
+    uint64_t *stk = ws + L.wave + (uint64_t)wv * L.per_wave;
+    uint64_t *X = stk + (uint64_t)a.levels * 2 * a.wmax;
+    Scal *scal = reinterpret_cast<Scal *>(X + a.wmax);
+    uint32_t *cand = reinterpret_cast<uint32_t *>(X + a.wmax + (uint64_t)a.levels * 2);
+    WaveAcc w;
+    uint64_t no_bytes = 0;
+    for (;;) {
+        if (tid == 0) ctl[0] = atomicAdd(a.qhead, 1u);
+        __syncthreads();
+        const uint32_t ti = ctl[0];
+        if (ti >= a.ntasks) break;
+        const uint64_t t0 = wall_clock64();
+        uint32_t task;
+        const Cur g = select_task(a, a.tasks[ti], w, task);
+        const uint32_t k = g.k, h0 = g.gran ? 2u : 1u;
+        if (wv == 0) {
+            const uint32_t r = stage_task(g, task, list, w.bytes);
+            if (lane == 0) ctl[1] = r;
+            ++w.ntask;
+        }
+        __syncthreads();
+        const uint32_t R = ctl[1], W = (R + 63) / 64;
+        if (g.gran && k == 3) {  // an edge task at k=3 adds |T| (no bitmap needed)
+            if (wv == 0) add_checked(w.run, R, w.run_ovf);
+            __syncthreads();
+            continue;
+        }
+        build_rows_block(g, list, offs, starts, rows, R, W, wv == 0 ? w.bytes : no_bytes);
+        // the root, by wave 0: a closed form, or the pivot and the branch list
+        if (wv == 0) {
+            uint32_t best_s = 0, best_j = INF32;
+            uint64_t sum_s = 0;
+            for (uint32_t c = lane; c < R; c += 64) {
+                const uint64_t *rc = rows + (uint64_t)c * W;
+                uint32_t sc = 0;
+                for (uint32_t x = 0; x < W; ++x) sc += (uint32_t)__popcll(rc[x]);
+                sum_s += sc;
+                if (best_j == INF32 || sc > best_s) {
+                    best_s = sc;
+                    best_j = c;
+                }
+            }
+            uint64_t key = best_j != INF32 ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
+            key = wave_max64(key);
+            uint32_t dummy = 0;
+            const uint64_t tot_s = wave_sum64_checked(sum_s, dummy);
+            const uint32_t max_s = (uint32_t)(key >> 32), pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
+            uint32_t nb = 0;
+            if (h0 + 1 >= k) {
+                add_checked(w.run, R, w.run_ovf);  // (edge tasks at k = 3 never get here)
+            } else if (h0 + 2 == k) {
+                add_checked(w.run, tot_s / 2, w.run_ovf);
+            } else if (h0 + max_s + 1 >= k && h0 + R >= k) {
+                // non-pivot branches: T & ~N(pv) without pv, ascending
+                const uint64_t *rp = rows + (uint64_t)pv * W;
+                uint32_t cnt = 0;
+                for (uint32_t x = 0; x < W; ++x) {
+                    const uint32_t lo = x * 64;
+                    const uint64_t all = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
+                    uint64_t bw = all & ~rp[x];
+                    if (x == (pv >> 6)) bw &= ~(1ull << (pv & 63));
+                    if (lane == 0) bnp[x] = bw;
+                    if ((bw >> lane) & 1ull) offs[cnt + (uint32_t)__popcll(bw & lanemask_lt())] = lo + lane;
+                    cnt += (uint32_t)__popcll(bw);
+                }
+                nb = cnt + 1;
+            }
+            if (lane == 0) {
+                ctl[2] = 0;
+                ctl[3] = nb;
+                ctl[4] = pv;
+            }
+        }
+        __syncthreads();
+        const uint32_t nb = ctl[3], pv = ctl[4];
+        for (;;) {
+            uint32_t i = 0;
+            if (lane == 0) i = atomicAdd(&ctl[2], 1u);
+            i = __shfl(i, 0, 64);
+            if (i >= nb) break;
+            // branch i: 0 = the pivot, i >= 1 = the i-th non-pivot vertex
+            const uint32_t b = i == 0 ? pv : offs[i - 1];
+            const uint64_t *rb = rows + (uint64_t)b * W;
+            for (uint32_t x = lane; x < W; x += 64) {
+                const uint32_t lo = x * 64;
+                const uint64_t all = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
+                uint64_t taken;  // non-pivot branches taken before this one
+                if (i == 0) taken = bnp[x];
+                else taken = x < (b >> 6) ? bnp[x] : x == (b >> 6) ? (bnp[x] & ((1ull << (b & 63)) - 1)) : 0ull;
+                X[x] = all & ~taken & rb[x];
+            }
+            const uint32_t h = i == 0 ? h0 : h0 + 1, p = i == 0 ? 1u : 0u;
+            add_checked(w.run,
+                        pivot_count<false, true>(a, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf, nullptr,
+                                                 p, false),
+                        w.run_ovf);
+        }
+        __syncthreads();
+        if (wv == 0) {
             const uint64_t dt = wall_clock64() - t0;
             w.tmax = dt > w.tmax ? dt : w.tmax;
             w.tsum += dt;
@@ -1509,13 +1715,36 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             L = make_layout(a.rmax, a.wmax, 0, 0, true);
             variant[b] = 2u;
         }
+        // large tasks of pivoting: blocks of several waves sharing one bitmap
+        uint32_t nw = 1;
+        WideLayout WL{};
+        if (pivot && !lc && c >= 3) {
+            bool fits = false;
+            if (c < NCLASS - 1)
+                for (uint32_t t = 16; t >= 2 && !fits; t >>= 1) {
+                    WL = make_wide_layout(a.rmax, a.wmax, a.levels, t);
+                    if (WL.total * 8 <= lds_cap) {
+                        nw = t;
+                        fits = true;
+                    }
+                }
+            global = !fits;
+            if (!fits) {
+                nw = 8;
+                WL = make_wide_layout(a.rmax, a.wmax, a.levels, nw);
+            }
+            kern = global ? reinterpret_cast<const void *>(&k_pivot_wide<true>)
+                          : reinterpret_cast<const void *>(&k_pivot_wide<false>);
+            L.total = WL.total;
+            variant[b] = global ? 5u : 4u;
+        }
         const size_t lds = global ? 0 : L.total * 8;
         if (!global) KC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int per_cu = 0;
-        KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+        KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(64 * nw), lds));
         if (per_cu < 1) per_cu = 1;
         uint64_t blocks = std::max<uint64_t>(1, (uint64_t)per_cu * ctx->cu / std::max<uint32_t>(1, grid_share));
-        a.chunk = (c == 0 && bcnt[b] > blocks * 64) ? 4 : 1;
+        a.chunk = (c == 0 && bcnt[b] > blocks * 64) ? 4 : 1;  // (the wide kernel takes one task per grab)
         blocks = std::min<uint64_t>(blocks, ((uint64_t)bcnt[b] + a.chunk - 1) / a.chunk);
         if (global) {
             a.gscr_words = L.total;
@@ -1528,7 +1757,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             a.gscr = scr.gscr.as<uint64_t>();
         }
         void *args[] = {&a};
-        KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64), args, lds, s));
+        KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, s));
         hipEventRecord(ev[b], s);
         launched[b] = true;
     }
