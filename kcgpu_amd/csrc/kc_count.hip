@@ -75,6 +75,7 @@ struct CountArgs {
     uint64_t *gscr;
     uint64_t gscr_words;
     uint64_t *local;        // per-vertex counts of the (single) query, or null
+    uint32_t chain;         // wide kernel: pivot-chain levels walked before the entries
 };
 
 // ---- workspace layout (u64 units) -----------------------------------------
@@ -1253,23 +1254,29 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
 // ---- the wide kernel: one block of NW waves per large task -----------------
 // Tasks of 257..1024 vertices (LDS) and above (HBM slice) hold a bitmap of up
 // to 128 KB, so a one-wave block leaves most of a CU idle.  Here the NW waves
-// of a block share one task: they build its bitmap together, wave 0 picks the
-// root pivot, and the root's branches -- the pivot branch first, then each
-// non-pivot vertex b with the candidates (T minus the non-pivot branches
-// before b) & N(b) -- are handed to the waves through an LDS counter; each
-// wave runs its subtree with its own stack (pivot_count<false, true>).
+// of a block share one task: they build its bitmap together, then wave 0
+// walks the SCT's pivot chain from the root (node j = the j-th pivot child:
+// h0 held, j pivots) down to a node of at most 64 candidates or to the chain
+// cap, and every non-pivot branch b hanging off chain node j -- candidates
+// (X_j minus the non-pivot branches of j before b) & N(b), h0+1 held, j
+// pivots -- plus the chain's last node become entries that the waves take
+// from an LDS counter and finish with their own stacks
+// (pivot_count<false, true>).  The chain is the part of the tree that one
+// wave would otherwise walk alone.
 struct WideLayout {
-    uint64_t ctl, list, offs, starts, rows, wave, per_wave, total;
+    uint64_t ctl, list, offs, starts, chain, rows, wave, per_wave, total;
 };
 
-__host__ __device__ inline WideLayout make_wide_layout(uint32_t rmax, uint32_t wmax, uint32_t levels, uint32_t nw)
+__host__ __device__ inline WideLayout make_wide_layout(uint32_t rmax, uint32_t wmax, uint32_t levels, uint32_t nw,
+                                                       uint32_t chain)
 {
     WideLayout L;
     L.ctl = 0;
     L.list = L.ctl + 8;
     L.offs = L.list + (rmax + 1) / 2;
     L.starts = L.offs + (rmax + 2) / 2;
-    L.rows = L.starts + (rmax + 1) / 2 + wmax;  // starts is followed by the root's branch set
+    L.chain = L.starts + (rmax + 1) / 2;         // per chain level: candidates, non-pivot branches
+    L.rows = L.chain + (uint64_t)chain * 2 * wmax;
     L.wave = L.rows + (uint64_t)rmax * wmax;
     L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2;
     L.total = L.wave + L.per_wave * nw;
@@ -1324,12 +1331,11 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
     extern __shared__ uint64_t lds[];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, NW = blockDim.x >> 6;
     uint64_t *ws = GLOBAL ? a.gscr + (uint64_t)blockIdx.x * a.gscr_words : lds;
-    const WideLayout L = make_wide_layout(a.rmax, a.wmax, a.levels, NW);
+    const WideLayout L = make_wide_layout(a.rmax, a.wmax, a.levels, NW, a.chain);
     uint32_t *ctl = reinterpret_cast<uint32_t *>(ws + L.ctl);  // task, R, next branch, branches, pivot
     uint32_t *list = reinterpret_cast<uint32_t *>(ws + L.list);
-    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the branch list
+    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the entries (j << 26 | b)
     uint32_t *starts = reinterpret_cast<uint32_t *>(ws + L.starts);
-    uint64_t *bnp = ws + L.rows - a.wmax;                        // non-pivot branch set of the root
     uint64_t *rows = ws + L.rows;
     uint64_t *stk = ws + L.wave + (uint64_t)wv * L.per_wave;
     uint64_t *X = stk + (uint64_t)a.levels * 2 * a.wmax;
@@ -1359,73 +1365,106 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
             continue;
         }
         build_rows_block(g, list, offs, starts, rows, R, W, wv == 0 ? w.bytes : no_bytes);
-        // the root, by wave 0: a closed form, or the pivot and the branch list
+        // the pivot chain, by wave 0: closed forms at the root, or the entries
+        uint64_t *chX = ws + L.chain, *chB = chX + a.wmax;  // level j at chX + 2 j wmax
         if (wv == 0) {
-            uint32_t best_s = 0, best_j = INF32;
-            uint64_t sum_s = 0;
-            for (uint32_t c = lane; c < R; c += 64) {
-                const uint64_t *rc = rows + (uint64_t)c * W;
-                uint32_t sc = 0;
-                for (uint32_t x = 0; x < W; ++x) sc += (uint32_t)__popcll(rc[x]);
-                sum_s += sc;
-                if (best_j == INF32 || sc > best_s) {
-                    best_s = sc;
-                    best_j = c;
-                }
+            uint32_t ne = 0, endj = INF32;
+            for (uint32_t x = lane; x < W; x += 64) {
+                const uint32_t lo = x * 64;
+                chX[x] = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
             }
-            uint64_t key = best_j != INF32 ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
-            key = wave_max64(key);
-            uint32_t dummy = 0;
-            const uint64_t tot_s = wave_sum64_checked(sum_s, dummy);
-            const uint32_t max_s = (uint32_t)(key >> 32), pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
-            uint32_t nb = 0;
-            if (h0 + 1 >= k) {
-                add_checked(w.run, R, w.run_ovf);  // (edge tasks at k = 3 never get here)
-            } else if (h0 + 2 == k) {
-                add_checked(w.run, tot_s / 2, w.run_ovf);
-            } else if (h0 + max_s + 1 >= k && h0 + R >= k) {
-                // non-pivot branches: T & ~N(pv) without pv, ascending
-                const uint64_t *rp = rows + (uint64_t)pv * W;
-                uint32_t cnt = 0;
+            sync_tree<true>();
+            for (uint32_t j = 0;; ++j) {
+                const uint64_t *Xj = chX + (uint64_t)j * 2 * a.wmax;
+                uint64_t *Bj = chB + (uint64_t)j * 2 * a.wmax;
+                const uint32_t p = j;
+                uint32_t cx = 0;
                 for (uint32_t x = 0; x < W; ++x) {
-                    const uint32_t lo = x * 64;
-                    const uint64_t all = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
-                    uint64_t bw = all & ~rp[x];
-                    if (x == (pv >> 6)) bw &= ~(1ull << (pv & 63));
-                    if (lane == 0) bnp[x] = bw;
-                    if ((bw >> lane) & 1ull) offs[cnt + (uint32_t)__popcll(bw & lanemask_lt())] = lo + lane;
-                    cnt += (uint32_t)__popcll(bw);
+                    const uint64_t xw = Xj[x];
+                    if ((xw >> lane) & 1ull) cand[cx + (uint32_t)__popcll(xw & lanemask_lt())] = x * 64 + lane;
+                    cx += (uint32_t)__popcll(xw);
                 }
-                nb = cnt + 1;
+                sync_tree<true>();
+                if (cx == 0) {
+                    if (k >= h0 && k - h0 <= p) {
+                        const uint64_t idx = (uint64_t)p * a.bstride + (k - h0);
+                        if (a.binom_ovf[idx]) w.run_ovf = 1;
+                        add_checked(w.run, a.binom[idx], w.run_ovf);
+                    }
+                    break;
+                }
+                if (h0 + p + cx < k) break;
+                if (cx <= 64 || j + 1 == a.chain) {  // the rest of the chain is one entry
+                    endj = j;
+                    break;
+                }
+                uint32_t best_s = 0, best_j = INF32;
+                uint64_t sum_s = 0;
+                for (uint32_t c = lane; c < cx; c += 64) {
+                    const uint32_t v = cand[c];
+                    const uint64_t *rv = rows + (uint64_t)v * W;
+                    uint32_t sc = 0;
+                    for (uint32_t x = 0; x < W; ++x) sc += (uint32_t)__popcll(Xj[x] & rv[x]);
+                    sum_s += sc;
+                    if (best_j == INF32 || sc > best_s) {
+                        best_s = sc;
+                        best_j = v;
+                    }
+                }
+                uint64_t key = best_j != INF32 ? (((uint64_t)best_s << 32) | (INF32 - best_j)) : 0ull;
+                key = wave_max64(key);
+                uint32_t dummy = 0;
+                const uint64_t tot_s = wave_sum64_checked(sum_s, dummy);
+                const uint32_t max_s = (uint32_t)(key >> 32), pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
+                if (j == 0 && h0 + 2 == k) {  // k-cliques of T with the root's h0 held: |E(T)|
+                    add_checked(w.run, tot_s / 2, w.run_ovf);
+                    break;
+                }
+                if (h0 + p + max_s + 1 < k) break;
+                const uint64_t *rp = rows + (uint64_t)pv * W;
+                uint64_t *Xn = chX + (uint64_t)(j + 1) * 2 * a.wmax;
+                for (uint32_t x = 0; x < W; ++x) {
+                    uint64_t bw = Xj[x] & ~rp[x];
+                    if (x == (pv >> 6)) bw &= ~(1ull << (pv & 63));
+                    if (lane == 0) {
+                        Bj[x] = bw;
+                        Xn[x] = Xj[x] & ~bw & rp[x];
+                    }
+                    if ((bw >> lane) & 1ull) offs[ne + (uint32_t)__popcll(bw & lanemask_lt())] = (j << 26) | (x * 64 + lane);
+                    ne += (uint32_t)__popcll(bw);
+                }
+                sync_tree<true>();
             }
             if (lane == 0) {
+                if (endj != INF32) offs[ne++] = (endj << 26) | 0x3FFFFFFu;
                 ctl[2] = 0;
-                ctl[3] = nb;
-                ctl[4] = pv;
+                ctl[3] = ne;
             }
         }
         __syncthreads();
-        const uint32_t nb = ctl[3], pv = ctl[4];
+        const uint32_t ne = ctl[3];
         for (;;) {
             uint32_t i = 0;
             if (lane == 0) i = atomicAdd(&ctl[2], 1u);
             i = __shfl(i, 0, 64);
-            if (i >= nb) break;
-            // branch i: 0 = the pivot, i >= 1 = the i-th non-pivot vertex
-            const uint32_t b = i == 0 ? pv : offs[i - 1];
-            const uint64_t *rb = rows + (uint64_t)b * W;
+            if (i >= ne) break;
+            const uint32_t e = offs[i], j = e >> 26, b = e & 0x3FFFFFFu;
+            const uint64_t *Xj = chX + (uint64_t)j * 2 * a.wmax, *Bj = chB + (uint64_t)j * 2 * a.wmax;
+            const bool end = (b == 0x3FFFFFFu);
+            const uint64_t *rb = rows + (uint64_t)(end ? 0 : b) * W;
             for (uint32_t x = lane; x < W; x += 64) {
-                const uint32_t lo = x * 64;
-                const uint64_t all = (R - lo >= 64) ? ~0ull : ((1ull << (R - lo)) - 1);
-                uint64_t taken;  // non-pivot branches taken before this one
-                if (i == 0) taken = bnp[x];
-                else taken = x < (b >> 6) ? bnp[x] : x == (b >> 6) ? (bnp[x] & ((1ull << (b & 63)) - 1)) : 0ull;
-                X[x] = all & ~taken & rb[x];
+                if (end) {
+                    X[x] = Xj[x];
+                } else {
+                    // the non-pivot branches of node j taken before b
+                    const uint64_t taken = x < (b >> 6) ? Bj[x] : x == (b >> 6) ? (Bj[x] & ((1ull << (b & 63)) - 1)) : 0ull;
+                    X[x] = Xj[x] & ~taken & rb[x];
+                }
             }
-            const uint32_t h = i == 0 ? h0 : h0 + 1, p = i == 0 ? 1u : 0u;
+            const uint32_t h = end ? h0 : h0 + 1;
             add_checked(w.run,
                         pivot_count<false, true>(a, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf, nullptr,
-                                                 p, false),
+                                                 j, false),
                         w.run_ovf);
         }
         __syncthreads();
@@ -1719,10 +1758,11 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         uint32_t nw = 1;
         WideLayout WL{};
         if (pivot && !lc && c >= 3) {
+            a.chain = c == 3 ? 32 : 16;
             bool fits = false;
             if (c < NCLASS - 1)
                 for (uint32_t t = 16; t >= 2 && !fits; t >>= 1) {
-                    WL = make_wide_layout(a.rmax, a.wmax, a.levels, t);
+                    WL = make_wide_layout(a.rmax, a.wmax, a.levels, t, a.chain);
                     if (WL.total * 8 <= lds_cap) {
                         nw = t;
                         fits = true;
@@ -1731,7 +1771,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             global = !fits;
             if (!fits) {
                 nw = 8;
-                WL = make_wide_layout(a.rmax, a.wmax, a.levels, nw);
+                WL = make_wide_layout(a.rmax, a.wmax, a.levels, nw, a.chain);
             }
             kern = global ? reinterpret_cast<const void *>(&k_pivot_wide<true>)
                           : reinterpret_cast<const void *>(&k_pivot_wide<false>);
