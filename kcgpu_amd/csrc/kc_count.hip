@@ -403,40 +403,6 @@ __device__ inline uint32_t tri_direct(uint64_t X, uint64_t row, uint64_t lane_ab
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
 }
 
-// 4-cliques of G[X], counted at their smallest vertex x: lane x counts the
-// triangles of its forward neighbourhood Y = X & N(x) & (> x) by Goodman's
-// identity inside Y, walking y in Y (degree of y in Y) and, for the
-// complement triangles of Y, the complement neighbours z of y inside Y.
-__device__ inline uint32_t k4_forward(uint64_t X, uint64_t row, bool mem, uint64_t lane_above)
-{
-    const uint64_t Y = mem ? (X & row & lane_above) : 0ull;
-    const uint32_t ny = (uint32_t)__popcll(Y);
-    uint32_t sq = 0, tb = 0;
-    uint64_t rest = Y;
-    while (__ballot(rest != 0)) {
-        const uint32_t y = rest ? (uint32_t)__builtin_ctzll(rest) : 0u;
-        const uint64_t ry = bperm64(row, y);
-        uint64_t cy = 0;
-        if (rest) {
-            const uint32_t dy = (uint32_t)__popcll(Y & ry);
-            sq += dy * (ny - 1 - dy);
-            cy = Y & ~ry & (y == 63 ? 0ull : (~0ull << (y + 1)));
-            rest &= rest - 1;
-        }
-        uint64_t r2 = cy;
-        while (__ballot(r2 != 0)) {
-            const uint32_t z = r2 ? (uint32_t)__builtin_ctzll(r2) : 0u;
-            const uint64_t rz = bperm64(row, z);
-            if (r2) {
-                tb += (uint32_t)__popcll(cy & ~rz & (z == 63 ? 0ull : (~0ull << (z + 1))));
-                r2 &= r2 - 1;
-            }
-        }
-    }
-    const uint32_t mine = ny >= 3 ? ny * (ny - 1) * (ny - 2) / 6 - sq / 2 - tb : 0u;
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(mine));
-}
-
 // X is a near-clique: no member has more complement neighbours above it than
 // two thirds of |X| (the complement walks stay short)
 __device__ inline bool near_clique(uint64_t X, uint64_t row, bool mem, uint64_t lane_above, uint32_t cx)
@@ -444,6 +410,69 @@ __device__ inline bool near_clique(uint64_t X, uint64_t row, bool mem, uint64_t 
     const uint32_t c = mem ? (uint32_t)__popcll(X & ~row & lane_above) : 0u;
     const uint32_t cmax = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32(c));
     return 3 * cmax < 2 * cx;
+}
+
+// |E|, triangles and 4-cliques of G[X] from its complement H (the non-edges
+// inside X), by inclusion-exclusion over the edge sets F of H that span at
+// most four vertices, sum_F (-1)^|F| C(n - |V(F)|, 4 - |V(F)|):
+//   c4 = C(n,4) - m C(n-2,2) + C(m,2) + (n-4) P3 - (n-3) K3 - P4 - S3 + C4 + paw - dia + K4
+// where, all as subgraphs of H: m edges, P3 = sum_x C(d_x,2) two-edge paths
+// (C(m,2) - P3 is the number of two-edge matchings), K3 triangles,
+// P4 = sum_{xy in E} (d_x-1)(d_y-1) - 3 K3 three-edge paths, S3 = sum_x C(d_x,3)
+// three-edge stars, C4 = 1/2 sum_{x<y} C(c_xy,2) four-cycles (c_xy = common
+// neighbours), paw = sum_x t_x (d_x-2) triangles with a pendant edge, dia =
+// sum_{xy in E} C(c_xy,2) four-cycles with a chord, K4 four-cliques; and
+// Goodman's c3 = C(n,3) - 1/2 sum_x d_x (n-1-d_x) - K3.  Lane x holds its
+// complement row Hx; one uniform walk over u in X (row u by v_readlane) hands
+// c_ux = |Hu & Hx| to every lane, and a walk over the H-edges u < v counts the
+// H-edges inside Hu & Hv above v (K4, rare in a near-clique).  The lane terms
+// are summed scaled by 6 so every share is an integer.  O(|X| + |E(H)|) VALU
+// steps with no cross-lane data waits.
+struct CliqueCounts {
+    uint32_t e, c3, c4;
+};
+
+__device__ inline CliqueCounts k34_complement(uint64_t X, uint64_t row, bool mem, uint32_t cx)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t Hl = mem ? (X & ~row & ~(1ull << lane)) : 0ull;
+    const int32_t d = (int32_t)__popcll(Hl), n = (int32_t)cx;
+    const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+    int32_t c4acc = 0, t2 = 0, p4acc = 0, dia = 0, k4acc = 0;
+    for (uint64_t bits = X; bits; bits &= bits - 1) {
+        const uint32_t u = (uint32_t)__builtin_ctzll(bits);
+        const uint64_t Hu = X & ~rdlane64(row, u) & ~(1ull << u);
+        const int32_t du = (int32_t)__popcll(Hu);
+        const int32_t c = (int32_t)__popcll(Hl & Hu);
+        const int32_t cc = c * (c - 1) / 2;
+        const bool above_u = lane > u;
+        c4acc += above_u ? cc : 0;
+        if ((Hu >> lane) & 1ull) {
+            t2 += c;
+            p4acc += above_u ? (du - 1) * (d - 1) : 0;
+            dia += above_u ? cc : 0;
+        }
+        for (uint64_t vb = Hu & (u == 63 ? 0ull : (~0ull << (u + 1))); vb; vb &= vb - 1) {
+            const uint32_t v = (uint32_t)__builtin_ctzll(vb);
+            const uint64_t S = Hu & ~rdlane64(row, v) & (v == 63 ? 0ull : (~0ull << (v + 1)));
+            if (__popcll(S) >= 2 && ((S >> lane) & 1ull)) k4acc += (int32_t)__popcll(S & Hl & lane_above);
+        }
+    }
+    const int32_t t = t2 / 2;  // H-triangles through the lane's vertex
+    const int32_t c4l = 6 * (n - 4) * (d * (d - 1) / 2) - 2 * (n - 3) * t - 6 * (p4acc - t) -
+                        (d * (d - 1) * (d - 2)) + 3 * c4acc + 6 * t * (d - 2) - 6 * dia + 6 * k4acc;
+    const int32_t c3l = -3 * d * (n - 1 - d) - 2 * t;
+    const int32_t s_d = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)d));
+    const int32_t s_4 = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)c4l));
+    const int32_t s_3 = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)c3l));
+    const int32_t m = s_d / 2;
+    const int32_t c4_6 = n * (n - 1) * (n - 2) * (n - 3) / 4 - 3 * m * (n - 2) * (n - 3) + 3 * m * (m - 1) + s_4;
+    const int32_t c3_6 = n * (n - 1) * (n - 2) + s_3;
+    CliqueCounts r;
+    r.e = (uint32_t)(n * (n - 1) / 2 - m);
+    r.c3 = (uint32_t)(c3_6 / 6);
+    r.c4 = (uint32_t)(c4_6 / 6);
+    return r;
 }
 
 __device__ inline uint64_t mul_chk(uint64_t x, uint64_t y, uint32_t &ovf)
@@ -519,17 +548,12 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
                 } else if (h + 4 == k && near_clique(X, row, mem, lane_above, cx)) {
                     // h = k-4 over a near-clique X: the subtree holds
                     // C(p,4) + C(p,3)|X| + C(p,2)|E(X)| + p c3(X) + c4(X) k-cliques
-                    const uint64_t cmpl = mem ? (X & ~row & lane_above) : 0ull;
-                    const uint32_t cmax =
-                        (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32((uint32_t)__popcll(cmpl)));
-                    const uint64_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(v)) / 2;
-                    const uint64_t c3 = tri_complement(row, cmpl, cmax, mem, v, cx);
-                    const uint64_t c4 = k4_forward(X, row, mem, lane_above);
+                    const CliqueCounts cc = k34_complement(X, row, mem, cx);
                     add_checked(acc, binom_chk(a, p, 4, ovf), ovf);
                     add_checked(acc, mul_chk(binom_chk(a, p, 3, ovf), cx, ovf), ovf);
-                    add_checked(acc, mul_chk(binom_chk(a, p, 2, ovf), e, ovf), ovf);
-                    add_checked(acc, mul_chk(p, c3, ovf), ovf);
-                    add_checked(acc, c4, ovf);
+                    add_checked(acc, mul_chk(binom_chk(a, p, 2, ovf), cc.e, ovf), ovf);
+                    add_checked(acc, mul_chk(p, cc.c3, ovf), ovf);
+                    add_checked(acc, cc.c4, ovf);
                 } else {
                     // (score + 1, 63 - index): the max names the smallest index of the largest score
                     uint32_t key = mem ? (((v + 1) << 8) | (63u - lane)) : 0u;
