@@ -422,12 +422,17 @@ __device__ inline bool near_clique(uint64_t X, uint64_t row, bool mem, uint64_t 
 // three-edge stars, C4 = 1/2 sum_{x<y} C(c_xy,2) four-cycles (c_xy = common
 // neighbours), paw = sum_x t_x (d_x-2) triangles with a pendant edge, dia =
 // sum_{xy in E} C(c_xy,2) four-cycles with a chord, K4 four-cliques; and
-// Goodman's c3 = C(n,3) - 1/2 sum_x d_x (n-1-d_x) - K3.  Lane x holds its
-// complement row Hx; one uniform walk over u in X (row u by v_readlane) hands
-// c_ux = |Hu & Hx| to every lane, and a walk over the H-edges u < v counts the
-// H-edges inside Hu & Hv above v (K4, rare in a near-clique).  The lane terms
-// are summed scaled by 6 so every share is an integer.  O(|X| + |E(H)|) VALU
-// steps with no cross-lane data waits.
+// Goodman's c3 = C(n,3) - 1/2 sum_x d_x (n-1-d_x) - K3.
+// Lane x holds its complement row Hx.  One uniform walk over u in X (row u by
+// v_readlane) hands c_ux = |Hu & Hx| to every lane, which keeps
+//   q  = sum_u c_ux^2,   qE = sum_{u in Hx} c_ux^2,   t2 = sum_{u in Hx} c_ux = 2 t_x,
+//   sd = sum_{u in Hx} d_u  (= sum_u c_ux),
+// so that, summed over lanes, 8 C4 = sum (q - d^2) - (sd - d), 2 P4 + 6 K3 =
+// sum (d-1)(sd-d), 4 dia = sum qE - t2.  The lane shares are summed scaled by
+// 24 so each is an integer.  A K4 of H has every vertex in at least three
+// H-triangles, so the K4 walk (over the H-edges u < v, counting the H-edges
+// inside Hu & Hv above v) only visits such vertices and is empty in most
+// near-cliques.
 struct CliqueCounts {
     uint32_t e, c3, c4;
 };
@@ -437,41 +442,51 @@ __device__ inline CliqueCounts k34_complement(uint64_t X, uint64_t row, bool mem
     const uint32_t lane = (uint32_t)lane_id();
     const uint64_t Hl = mem ? (X & ~row & ~(1ull << lane)) : 0ull;
     const int32_t d = (int32_t)__popcll(Hl), n = (int32_t)cx;
-    const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
-    int32_t c4acc = 0, t2 = 0, p4acc = 0, dia = 0, k4acc = 0;
+    uint32_t q = 0, qE = 0, t2 = 0, sd = 0;
     for (uint64_t bits = X; bits; bits &= bits - 1) {
         const uint32_t u = (uint32_t)__builtin_ctzll(bits);
         const uint64_t Hu = X & ~rdlane64(row, u) & ~(1ull << u);
-        const int32_t du = (int32_t)__popcll(Hu);
-        const int32_t c = (int32_t)__popcll(Hl & Hu);
-        const int32_t cc = c * (c - 1) / 2;
-        const bool above_u = lane > u;
-        c4acc += above_u ? cc : 0;
+        const uint32_t du = (uint32_t)__popcll(Hu);
+        const uint32_t c = (uint32_t)__popcll(Hl & Hu);
+        const uint32_t c2 = c * c;
+        q += c2;
         if ((Hu >> lane) & 1ull) {
+            qE += c2;
             t2 += c;
-            p4acc += above_u ? (du - 1) * (d - 1) : 0;
-            dia += above_u ? cc : 0;
-        }
-        for (uint64_t vb = Hu & (u == 63 ? 0ull : (~0ull << (u + 1))); vb; vb &= vb - 1) {
-            const uint32_t v = (uint32_t)__builtin_ctzll(vb);
-            const uint64_t S = Hu & ~rdlane64(row, v) & (v == 63 ? 0ull : (~0ull << (v + 1)));
-            if (__popcll(S) >= 2 && ((S >> lane) & 1ull)) k4acc += (int32_t)__popcll(S & Hl & lane_above);
+            sd += du;
         }
     }
-    const int32_t t = t2 / 2;  // H-triangles through the lane's vertex
-    const int32_t c4l = 6 * (n - 4) * (d * (d - 1) / 2) - 2 * (n - 3) * t - 6 * (p4acc - t) -
-                        (d * (d - 1) * (d - 2)) + 3 * c4acc + 6 * t * (d - 2) - 6 * dia + 6 * k4acc;
+    const int32_t t = (int32_t)(t2 / 2);  // H-triangles through the lane's vertex
+    // K4 of H among the vertices in >= 3 H-triangles
+    int32_t k4acc = 0;
+    const uint64_t T3 = __ballot(t >= 3);
+    if (__popcll(T3) >= 4) {
+        const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+        for (uint64_t ub = T3; ub; ub &= ub - 1) {
+            const uint32_t u = (uint32_t)__builtin_ctzll(ub);
+            const uint64_t Hu = T3 & ~rdlane64(row, u) & (u == 63 ? 0ull : (~0ull << (u + 1)));
+            for (uint64_t vb = Hu; vb; vb &= vb - 1) {
+                const uint32_t v = (uint32_t)__builtin_ctzll(vb);
+                const uint64_t S = Hu & ~rdlane64(row, v) & (v == 63 ? 0ull : (~0ull << (v + 1)));
+                if (__popcll(S) >= 2 && ((S >> lane) & 1ull)) k4acc += (int32_t)__popcll(S & Hl & lane_above);
+            }
+        }
+    }
+    const int32_t iq = (int32_t)q, iqE = (int32_t)qE, isd = (int32_t)sd, it2 = (int32_t)t2;
+    const int32_t c4l = 12 * (n - 4) * d * (d - 1) - 8 * (n - 3) * t - 12 * (d - 1) * (isd - d) + 24 * t -
+                        4 * d * (d - 1) * (d - 2) + 3 * ((iq - d * d) - (isd - d)) + 24 * t * (d - 2) -
+                        6 * (iqE - it2) + 24 * k4acc;
     const int32_t c3l = -3 * d * (n - 1 - d) - 2 * t;
     const int32_t s_d = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)d));
     const int32_t s_4 = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)c4l));
     const int32_t s_3 = (int32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32((uint32_t)c3l));
     const int32_t m = s_d / 2;
-    const int32_t c4_6 = n * (n - 1) * (n - 2) * (n - 3) / 4 - 3 * m * (n - 2) * (n - 3) + 3 * m * (m - 1) + s_4;
+    const int32_t c4_24 = n * (n - 1) * (n - 2) * (n - 3) - 12 * m * (n - 2) * (n - 3) + 12 * m * (m - 1) + s_4;
     const int32_t c3_6 = n * (n - 1) * (n - 2) + s_3;
     CliqueCounts r;
     r.e = (uint32_t)(n * (n - 1) / 2 - m);
     r.c3 = (uint32_t)(c3_6 / 6);
-    r.c4 = (uint32_t)(c4_6 / 6);
+    r.c4 = (uint32_t)(c4_24 / 24);
     return r;
 }
 
