@@ -890,15 +890,62 @@ __device__ uint64_t compact_and_count(const CountArgs &a, uint32_t k, const uint
     return sct64(a, k, row1, X1, h, p, nodes, ovf);
 }
 
+// Compaction to two words: once a node of the four-word kernel has 65..128
+// candidates, its subtree is re-encoded as a subproblem of at most 128
+// vertices (vertex j = the j-th member of X) with two-word rows in crows and
+// finished by the two-word register SCT, whose scores and closed forms cost a
+// quarter of the four-word ones.  Lanes l and l+64 load the full rows of
+// members l and l+64; word t of new row j is the ballot over lanes of bit x_j
+// of the rows of members 64t + lane (the bitmap is symmetric).
 template <int S>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
-                              uint32_t *idx, uint64_t &nodes, uint32_t &ovf)
+                              uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows);
+
+template <int S>
+__device__ uint64_t compact2_and_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t W,
+                                       const uint64_t (&X)[S], uint32_t *idx, uint64_t *crows, uint32_t h, uint32_t p,
+                                       uint64_t &nodes, uint32_t &ovf)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t lt = lanemask_lt();
+    uint32_t cx = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if ((X[s] >> lane) & 1ull) idx[cx + (uint32_t)__popcll(X[s] & lt)] = 64u * s + lane;
+        cx += (uint32_t)__popcll(X[s]);
+    }
+    __syncthreads();
+    const bool hi_ok = lane + 64 < cx;
+    const uint32_t m0 = idx[lane], m1 = hi_ok ? idx[lane + 64] : 0u;  // cx > 64: every lane has a first member
+    uint64_t o0[S], o1[S];
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        o0[w] = (uint32_t)w < W ? rows[(uint64_t)m0 * W + w] : 0ull;
+        o1[w] = (hi_ok && (uint32_t)w < W) ? rows[(uint64_t)m1 * W + w] : 0ull;
+    }
+    __syncthreads();  // idx is reused by the two-word kernel
+    uint32_t j = 0;
+#pragma unroll
+    for (int w = 0; w < S; ++w) {
+        for (uint64_t bits = X[w]; bits; bits &= bits - 1, ++j) {
+            const uint32_t bj = (uint32_t)__builtin_ctzll(bits);
+            const uint64_t c0 = __ballot((o0[w] >> bj) & 1ull), c1 = __ballot((o1[w] >> bj) & 1ull);
+            if (lane < 2) crows[2 * j + lane] = lane ? c1 : c0;
+        }
+    }
+    __syncthreads();
+    return pivot_reg<2>(a, k, crows, cx, h, idx, nodes, ovf, p, nullptr);
+}
+
+template <int S>
+__device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
+                              uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows)
 {
     const uint32_t lane = (uint32_t)lane_id();
     const uint32_t W = (R + 63) / 64;
     if (S == 1) {
         const uint64_t r1 = lane < R ? rows[lane] : 0ull;
-        return sct64(a, k, r1, R >= 64 ? ~0ull : ((1ull << R) - 1), h0, 0, nodes, ovf);
+        return sct64(a, k, r1, R >= 64 ? ~0ull : ((1ull << R) - 1), h0, p0, nodes, ovf);
     }
     uint64_t row[S][S];
 #pragma unroll
@@ -917,7 +964,7 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
     }
     int t = -1;
     bool pending = true;
-    uint32_t h = h0, p = 0;
+    uint32_t h = h0, p = p0;
     uint64_t acc = 0, my_nodes = 0;
     for (;;) {
         if (pending) {
@@ -925,7 +972,10 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
             uint32_t cx = 0;
 #pragma unroll
             for (int w = 0; w < S; ++w) cx += (uint32_t)__popcll(X[w]);
-            if (cx <= 64 && cx > 0 && h + 3 <= k && h + p + cx >= k) {
+            if (S == 4 && cx > 64 && cx <= 128 && h + 3 <= k && h + p + cx >= k) {
+                // a subtree of at most 128 candidates: two words per row from here on
+                add_checked(acc, compact2_and_count<S>(a, k, rows, W, X, idx, crows, h, p, nodes, ovf), ovf);
+            } else if (cx <= 64 && cx > 0 && h + 3 <= k && h + p + cx >= k) {
                 // a subtree of at most 64 candidates: one word per row from here on
                 add_checked(acc, compact_and_count<S>(a, k, rows, W, X, idx, h, p, nodes, ovf), ovf);
             } else if ((++my_nodes, cx == 0)) {
@@ -1165,6 +1215,7 @@ void k_pivot_reg(CountArgs a)
     uint32_t *offs = reinterpret_cast<uint32_t *>(lds + L.offs);
     uint32_t *starts = reinterpret_cast<uint32_t *>(lds + L.starts);
     uint64_t *rows = lds + L.rows;
+    uint64_t *crows = lds + L.total;  // (S = 4) the two-word rows of a compacted subtree: 128 x 2 words
     WaveAcc w;
     for (;;) {
         uint32_t start = 0;
@@ -1185,7 +1236,8 @@ void k_pivot_reg(CountArgs a)
             }
             __syncthreads();
             build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
-            add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf), w.run_ovf);
+            add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf, 0u, crows),
+                        w.run_ovf);
             __syncthreads();
             const uint64_t dt = wall_clock64() - t0;
             w.tmax = dt > w.tmax ? dt : w.tmax;
@@ -1791,6 +1843,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         if (pivot && !global && c <= 2 && !lc) {
             kern = register_kernel(c);
             L = make_layout(a.rmax, a.wmax, 0, 0, true);
+            if (c == 2) L.total += 256;  // the two-word rows of a compacted subtree
             variant[b] = 2u;
         }
         // large tasks of pivoting: blocks of several waves sharing one bitmap
