@@ -890,6 +890,81 @@ __device__ uint64_t compact_and_count(const CountArgs &a, uint32_t k, const uint
     return sct64(a, k, row1, X1, h, p, nodes, ovf);
 }
 
+// Triangles of G[X] for X of up to 64*S vertices (rows of stride W in LDS) by
+// lane-parallel forward walks.  Lane l takes the members l, 127-l, 128+l and
+// 255-l of X (in index order; the pairs balance the walks) and, for each
+// member x, walks its forward neighbours F = N(x) & X above x lowest first:
+// popping j leaves in F exactly the forward neighbours above j, so
+// |F & N(j)| counts the triangles x < j < y once.  When more than half the
+// pairs of X are edges the walk runs in the complement H instead (F = the
+// non-neighbours above x) and Goodman's identity
+//     t(G) = C(n,3) - 1/2 sum_x d_x (n-1-d_x) - t(H)
+// gives the count; either way a lane walks min(density, 1-density) * n
+// neighbours per member.  e2 = 2|E(X)|.  idx takes the member list.
+template <int S>
+__device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (&X)[S], uint32_t cx, uint32_t e2,
+                              uint32_t *idx)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t lt = lanemask_lt();
+    uint32_t before = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if ((X[s] >> lane) & 1ull) idx[before + (uint32_t)__popcll(X[s] & lt)] = 64u * s + lane;
+        before += (uint32_t)__popcll(X[s]);
+    }
+    __syncthreads();
+    const bool comp = e2 > cx * (cx - 1) / 2;  // |E| > C(n,2)/2
+    const uint64_t flip = comp ? ~0ull : 0ull;
+    constexpr int NR = S > 2 ? 4 : 2;  // member slots per lane
+    uint32_t val = 0, cnt = 0;
+    int r = -1;
+    uint64_t F[S];
+#pragma unroll
+    for (int w = 0; w < S; ++w) F[w] = 0;
+    for (;;) {
+        bool empty = true;
+#pragma unroll
+        for (int w = 0; w < S; ++w) empty = empty && F[w] == 0;
+        while (empty && r < NR - 1) {  // the lane's next member
+            ++r;
+            const uint32_t m = (r & 1) ? (128u * (uint32_t)(r >> 1) + 127u - lane) : (128u * (uint32_t)(r >> 1) + lane);
+            if (m >= cx) continue;
+            const uint32_t x = idx[m], xw = x >> 6, xb = x & 63;
+            uint32_t dg = 0;
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                const uint64_t rw = (uint32_t)w < W ? rows[(uint64_t)x * W + w] : 0ull;
+                dg += (uint32_t)__popcll(X[w] & rw);
+                const uint64_t above = (uint32_t)w > xw ? ~0ull : ((uint32_t)w < xw || xb == 63) ? 0ull : (~0ull << (xb + 1));
+                F[w] = X[w] & (rw ^ flip) & above;
+                empty = empty && F[w] == 0;
+            }
+            if (comp) val += dg * (cx - 1 - dg);
+        }
+        if (!__ballot(!empty)) break;
+        if (!empty) {
+            uint32_t j = 0;
+            bool found = false;
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                if (!found && F[w]) {
+                    j = 64u * w + (uint32_t)__builtin_ctzll(F[w]);
+                    F[w] &= F[w] - 1;
+                    found = true;
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < S; ++w)
+                if ((uint32_t)w < W) cnt += (uint32_t)__popcll(F[w] & (rows[(uint64_t)j * W + w] ^ flip));
+        }
+    }
+    val += 2 * cnt;
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(val));
+    __syncthreads();  // idx is reused
+    return comp ? cx * (cx - 1) * (cx - 2) / 6 - tot / 2 : tot / 2;
+}
+
 // Compaction to two words: once a node of the four-word kernel has 65..128
 // candidates, its subtree is re-encoded as a subproblem of at most 128
 // vertices (vertex j = the j-th member of X) with two-word rows in crows and
@@ -1013,43 +1088,8 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                         add_checked(acc, pp * cx, ovf);
                         add_checked(acc, e2 / 2, ovf);
                     } else {
-                        // h = k-3: C(p,3) + C(p,2)|X| + p|E(X)| + triangles(X); each
-                        // triangle i < j < l is counted once, at its smallest vertex
-                        // rows i live in slot i/64 of lane i%64: fetched with v_readlane
-                        // from a register chosen at compile time (wi is unrolled)
-                        uint32_t tri = 0;
-                        const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
-#pragma unroll
-                        for (int wi = 0; wi < S; ++wi) {
-                            uint64_t bits = X[wi];
-                            while (bits) {
-                                const uint32_t il = (uint32_t)__builtin_ctzll(bits);
-                                bits &= bits - 1;
-                                uint64_t Y[S];
-                                uint32_t ny = 0;
-#pragma unroll
-                                for (int w = 0; w < S; ++w) {
-                                    if (w < wi) {
-                                        Y[w] = 0;
-                                        continue;
-                                    }
-                                    const uint64_t above = (w > wi) ? ~0ull : (il == 63 ? 0ull : (~0ull << (il + 1)));
-                                    Y[w] = X[w] & rdlane64(row[wi][w], il) & above;
-                                    ny += (uint32_t)__popcll(Y[w]);
-                                }
-                                if (ny < 2) continue;
-#pragma unroll
-                                for (int sl = wi; sl < S; ++sl) {
-                                    if (!((Y[sl] >> lane) & 1ull)) continue;
-                                    uint32_t v = 0;
-#pragma unroll
-                                    for (int w = sl; w < S; ++w)
-                                        v += (uint32_t)__popcll(Y[w] & row[sl][w] & (w > sl ? ~0ull : lane_above));
-                                    tri += v;
-                                }
-                            }
-                        }
-                        tri = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
+                        // h = k-3: C(p,3) + C(p,2)|X| + p|E(X)| + triangles(X)
+                        const uint32_t tri = tri_lanes<S>(rows, W, X, cx, e2, idx);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) * (pp >= 2 ? pp - 2 : 0) / 6, ovf);
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2 * cx, ovf);
                         add_checked(acc, pp * (e2 / 2), ovf);
