@@ -356,6 +356,16 @@ __device__ inline uint64_t bperm64(uint64_t v, uint32_t src)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// A value every lane holds alike, moved to scalar registers: the compiler then
+// treats what is derived from it as uniform (scalar ALU, scalar branches)
+// instead of masking the wave lane by lane.
+__device__ inline uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ inline uint64_t uni64(uint64_t v)
+{
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
+
 __device__ inline uint64_t rdlane64(uint64_t v, uint32_t src)
 {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
@@ -516,11 +526,12 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
 {
     const uint32_t lane = (uint32_t)lane_id();
     const uint64_t lane_above = lane == 63 ? 0ull : (~0ull << (lane + 1));
-    uint64_t X = X0, stkC = 0, stkB = 0;
+    k = uni32(k);
+    uint64_t X = uni64(X0), stkC = 0, stkB = 0;
     uint32_t stkM = 0;
     int t = -1;
     bool pending = true;
-    uint32_t h = h0, p = p0;
+    uint32_t h = uni32(h0), p = uni32(p0);
     uint64_t acc = 0, my_nodes = 0;
     for (;;) {
         if (pending) {
@@ -1017,6 +1028,10 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                               uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows)
 {
     const uint32_t lane = (uint32_t)lane_id();
+    k = uni32(k);
+    R = uni32(R);
+    h0 = uni32(h0);
+    p0 = uni32(p0);
     const uint32_t W = (R + 63) / 64;
     if (S == 1) {
         const uint64_t r1 = lane < R ? rows[lane] : 0ull;
@@ -1260,7 +1275,7 @@ void k_pivot_reg(CountArgs a)
     for (;;) {
         uint32_t start = 0;
         if (lane == 0) start = atomicAdd(a.qhead, a.chunk);
-        start = __shfl(start, 0, 64);
+        start = uni32(start);
         if (start >= a.ntasks) break;
         const uint32_t stop = min(a.ntasks, start + a.chunk);
         for (uint32_t ti = start; ti < stop; ++ti) {
@@ -1329,7 +1344,7 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
     for (;;) {
         uint32_t start = 0;
         if (lane == 0) start = atomicAdd(a.qhead, a.chunk);
-        start = __shfl(start, 0, 64);
+        start = uni32(start);
         if (start >= a.ntasks) break;
         const uint32_t stop = min(a.ntasks, start + a.chunk);
         for (uint32_t ti = start; ti < stop; ++ti) {
@@ -1577,7 +1592,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
         for (;;) {
             uint32_t i = 0;
             if (lane == 0) i = atomicAdd(&ctl[2], 1u);
-            i = __shfl(i, 0, 64);
+            i = uni32(i);
             if (i >= ne) break;
             const uint32_t e = offs[i], j = e >> 26, b = e & 0x3FFFFFFu;
             const uint64_t *Xj = chX + (uint64_t)j * 2 * a.wmax, *Bj = chB + (uint64_t)j * 2 * a.wmax;
