@@ -515,6 +515,18 @@ __device__ inline uint64_t binom_chk(const CountArgs &a, uint32_t n, uint32_t r,
     return a.binom[idx];
 }
 
+// k-cliques of a subtree at h = k-3: C(p,3) + C(p,2)|X| + p|E(X)| + t(X)
+__device__ inline uint64_t close_k3(const CountArgs &a, uint32_t p, uint32_t cx, uint64_t e, uint64_t tri,
+                                    uint32_t &ovf)
+{
+    uint64_t r = 0;
+    add_checked(r, binom_chk(a, p, 3, ovf), ovf);
+    add_checked(r, mul_chk(binom_chk(a, p, 2, ovf), cx, ovf), ovf);
+    add_checked(r, mul_chk(p, e, ovf), ovf);
+    add_checked(r, tri, ovf);
+    return r;
+}
+
 // SCT of a subproblem of at most 64 vertices held one row per lane: lane i
 // keeps row i (bit j <=> vertex j of the subproblem is adjacent), X0 is the
 // candidate set, h0/p0 the held and pivot counts above it.  Every set is one
@@ -671,10 +683,29 @@ __device__ inline void sync_tree()
 // The SCT from the root candidate set (all R task vertices, held h0, no
 // pivots), or -- with init_x false -- from the candidate set already in X
 // with h0 held and p0 pivot vertices (a subtree handed to one wave of a block).
+template <int S>
+__device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (&X)[S], uint32_t cx, uint32_t e2,
+                              uint32_t *idx);
+template <int S>
+__device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
+                              uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows);
+
+// triangles of G[X] with X in LDS (W words): the lane walks with the words in registers
+template <int S>
+__device__ uint32_t tri_lds(const uint64_t *rows, uint32_t W, const uint64_t *X, uint32_t cx, uint32_t e2,
+                            uint32_t *idx)
+{
+    uint64_t Xr[S];
+#pragma unroll
+    for (int w = 0; w < S; ++w) Xr[w] = (uint32_t)w < W ? uni64(X[w]) : 0ull;
+    return tri_lanes<S>(rows, W, Xr, cx, e2, idx);
+}
+
 template <bool LOCAL, bool WIDE = false>
 __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
                                 Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
-                                uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true)
+                                uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true,
+                                uint64_t *crows2 = nullptr)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -747,6 +778,21 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                 }
                 add_checked(acc, sct64(a, k, row1, cx >= 64 ? ~0ull : ((1ull << cx) - 1), h, p, nodes, ovf), ovf);
                 --my_nodes;  // counted again as the subtree's root
+            } else if (!LOCAL && crows2 && cx <= 128 && h + 3 <= k && h + p + cx >= k) {
+                // 65..128 candidates: re-encode two words per row (word t of new
+                // row j = ballot of bit cand[j] over the rows of candidates
+                // 64t + lane) and finish the subtree in the two-word register SCT
+                const uint32_t m0 = cand[lane], m1 = lane + 64 < cx ? cand[lane + 64] : 0u;
+                for (uint32_t j = 0; j < cx; ++j) {
+                    const uint32_t xj = cand[j];
+                    const uint64_t w0 = rows[(uint64_t)m0 * W + (xj >> 6)];
+                    const uint64_t w1 = lane + 64 < cx ? rows[(uint64_t)m1 * W + (xj >> 6)] : 0ull;
+                    const uint64_t c0 = __ballot((w0 >> (xj & 63)) & 1ull), c1 = __ballot((w1 >> (xj & 63)) & 1ull);
+                    if (lane < 2) crows2[2 * j + lane] = lane ? c1 : c0;
+                }
+                sync_tree<WIDE>();
+                add_checked(acc, pivot_reg<2>(a, k, crows2, cx, h, cand, nodes, ovf, p, nullptr), ovf);
+                --my_nodes;
             } else if (h + p + cx >= k) {
                 // scores |X & row_j| for j in X: P lanes per candidate
                 uint32_t best_s = 0, best_j = INF32;
@@ -776,6 +822,13 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                     add_checked(acc, (uint64_t)p * cx, ovf);
                     add_checked(acc, tot_s / 2, ovf);
                     if (LOCAL) credit_hp(acc - before, (p ? (uint64_t)p - 1 : 0) + cx);
+                } else if (!LOCAL && h + 3 == k && W <= 16) {
+                    // h = k-3: closed form with the triangles of G[X] (lane walks)
+                    const uint32_t e2 = (uint32_t)tot_s;
+                    const uint32_t tri = W <= 4 ? tri_lds<4>(rows, W, X, cx, e2, cand)
+                                         : W <= 8 ? tri_lds<8>(rows, W, X, cx, e2, cand)
+                                                  : tri_lds<16>(rows, W, X, cx, e2, cand);
+                    add_checked(acc, close_k3(a, p, cx, tot_s / 2, tri, ovf), ovf);
                 } else if (h + p + max_s + 1 >= k) {
                     const uint32_t pv = INF32 - (uint32_t)(key & 0xFFFFFFFFull);
                     ++t;
@@ -881,12 +934,12 @@ __device__ uint64_t compact_and_count(const CountArgs &a, uint32_t k, const uint
         before += (uint32_t)__popcll(X[s]);
     }
     cx = before;
-    __syncthreads();
+    sync_tree<true>();
     uint64_t o[S];
     const uint32_t mine = lane < cx ? idx[lane] : 0u;
 #pragma unroll
     for (int w = 0; w < S; ++w) o[w] = (lane < cx && (uint32_t)w < W) ? rows[(uint64_t)mine * W + w] : 0ull;
-    __syncthreads();  // idx is reused by the next compaction
+    sync_tree<true>();  // idx is reused by the next compaction
     uint64_t row1 = 0;
     uint32_t j = 0;
 #pragma unroll
@@ -902,8 +955,8 @@ __device__ uint64_t compact_and_count(const CountArgs &a, uint32_t k, const uint
 }
 
 // Triangles of G[X] for X of up to 64*S vertices (rows of stride W in LDS) by
-// lane-parallel forward walks.  Lane l takes the members l, 127-l, 128+l and
-// 255-l of X (in index order; the pairs balance the walks) and, for each
+// lane-parallel forward walks.  Lane l takes the members l, 127-l, 128+l,
+// 255-l, ... of X (in index order; the pairs balance the walks) and, for each
 // member x, walks its forward neighbours F = N(x) & X above x lowest first:
 // popping j leaves in F exactly the forward neighbours above j, so
 // |F & N(j)| counts the triangles x < j < y once.  When more than half the
@@ -924,10 +977,10 @@ __device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (
         if ((X[s] >> lane) & 1ull) idx[before + (uint32_t)__popcll(X[s] & lt)] = 64u * s + lane;
         before += (uint32_t)__popcll(X[s]);
     }
-    __syncthreads();
+    sync_tree<true>();
     const bool comp = e2 > cx * (cx - 1) / 2;  // |E| > C(n,2)/2
     const uint64_t flip = comp ? ~0ull : 0ull;
-    constexpr int NR = S > 2 ? 4 : 2;  // member slots per lane
+    constexpr int NR = S > 2 ? S : 2;  // member slots per lane
     uint32_t val = 0, cnt = 0;
     int r = -1;
     uint64_t F[S];
@@ -972,7 +1025,7 @@ __device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (
     }
     val += 2 * cnt;
     const uint32_t tot = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(val));
-    __syncthreads();  // idx is reused
+    sync_tree<true>();  // idx is reused
     return comp ? cx * (cx - 1) * (cx - 2) / 6 - tot / 2 : tot / 2;
 }
 
@@ -1000,7 +1053,7 @@ __device__ uint64_t compact2_and_count(const CountArgs &a, uint32_t k, const uin
         if ((X[s] >> lane) & 1ull) idx[cx + (uint32_t)__popcll(X[s] & lt)] = 64u * s + lane;
         cx += (uint32_t)__popcll(X[s]);
     }
-    __syncthreads();
+    sync_tree<true>();
     const bool hi_ok = lane + 64 < cx;
     const uint32_t m0 = idx[lane], m1 = hi_ok ? idx[lane + 64] : 0u;  // cx > 64: every lane has a first member
     uint64_t o0[S], o1[S];
@@ -1009,7 +1062,7 @@ __device__ uint64_t compact2_and_count(const CountArgs &a, uint32_t k, const uin
         o0[w] = (uint32_t)w < W ? rows[(uint64_t)m0 * W + w] : 0ull;
         o1[w] = (hi_ok && (uint32_t)w < W) ? rows[(uint64_t)m1 * W + w] : 0ull;
     }
-    __syncthreads();  // idx is reused by the two-word kernel
+    sync_tree<true>();  // idx is reused by the two-word kernel
     uint32_t j = 0;
 #pragma unroll
     for (int w = 0; w < S; ++w) {
@@ -1019,7 +1072,7 @@ __device__ uint64_t compact2_and_count(const CountArgs &a, uint32_t k, const uin
             if (lane < 2) crows[2 * j + lane] = lane ? c1 : c0;
         }
     }
-    __syncthreads();
+    sync_tree<true>();
     return pivot_reg<2>(a, k, crows, cx, h, idx, nodes, ovf, p, nullptr);
 }
 
@@ -1424,7 +1477,7 @@ __host__ __device__ inline WideLayout make_wide_layout(uint32_t rmax, uint32_t w
     L.chain = L.starts + (rmax + 1) / 2;         // per chain level: candidates, non-pivot branches
     L.rows = L.chain + (uint64_t)chain * 2 * wmax;
     L.wave = L.rows + (uint64_t)rmax * wmax;
-    L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2;
+    L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2 + 256;  // + two-word rows
     L.total = L.wave + L.per_wave * nw;
     return L;
 }
@@ -1487,6 +1540,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
     uint64_t *X = stk + (uint64_t)a.levels * 2 * a.wmax;
     Scal *scal = reinterpret_cast<Scal *>(X + a.wmax);
     uint32_t *cand = reinterpret_cast<uint32_t *>(X + a.wmax + (uint64_t)a.levels * 2);
+    uint64_t *crows2 = X + a.wmax + (uint64_t)a.levels * 2 + (a.rmax + 1) / 2;  // 128 x 2 words
     WaveAcc w;
     uint64_t no_bytes = 0;
     for (;;) {
@@ -1610,7 +1664,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
             const uint32_t h = end ? h0 : h0 + 1;
             add_checked(w.run,
                         pivot_count<false, true>(a, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf, nullptr,
-                                                 j, false),
+                                                 j, false, crows2),
                         w.run_ovf);
         }
         __syncthreads();
