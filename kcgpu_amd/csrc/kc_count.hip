@@ -92,9 +92,9 @@ __host__ __device__ inline Layout make_layout(uint32_t rmax, uint32_t wmax, uint
     Layout L;
     L.misc = 0;
     L.list = L.misc + 8;
-    L.offs = L.list + (rmax + 1) / 2;
-    L.starts = L.offs + (rmax + 2) / 2;
-    L.rows = L.starts + (rmax + 1) / 2;
+    L.starts = L.list + (rmax + 1) / 2;  // list and starts are dead once the rows are built
+    L.offs = L.starts + (rmax + 1) / 2;
+    L.rows = L.offs + (rmax + 2) / 2;
     L.stk = L.rows + (uint64_t)rmax * wmax;
     const uint64_t sets = local ? 4 : 2;  // per level: candidates, branches (+ held, pivots)
     if (pivot) {
@@ -413,14 +413,6 @@ __device__ inline uint32_t tri_direct(uint64_t X, uint64_t row, uint64_t lane_ab
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(tri));
 }
 
-// X is a near-clique: no member has more complement neighbours above it than
-// two thirds of |X| (the complement walks stay short)
-__device__ inline bool near_clique(uint64_t X, uint64_t row, bool mem, uint64_t lane_above, uint32_t cx)
-{
-    const uint32_t c = mem ? (uint32_t)__popcll(X & ~row & lane_above) : 0u;
-    const uint32_t cmax = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_max_u32(c));
-    return 3 * cmax < 2 * cx;
-}
 
 // |E|, triangles and 4-cliques of G[X] from its complement H (the non-edges
 // inside X), by inclusion-exclusion over the edge sets F of H that span at
@@ -532,7 +524,8 @@ __device__ inline uint64_t close_k3(const CountArgs &a, uint32_t p, uint32_t cx,
 // candidate set, h0/p0 the held and pivot counts above it.  Every set is one
 // uniform 64-bit word, a score is one AND + popcount, a branch row is one
 // v_readlane pair, and the stack lives across the lanes (lane t = level t).
-// Nodes with h >= k-3 close in form as in pivot_reg.
+// Nodes with h >= k-3 close in form as in pivot_reg, nodes with h = k-4 by
+// k34_complement.
 __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t X0, uint32_t h0, uint32_t p0,
                           uint64_t &nodes, uint32_t &ovf)
 {
@@ -583,8 +576,8 @@ __device__ uint64_t sct64(const CountArgs &a, uint32_t k, uint64_t row, uint64_t
                         add_checked(acc, pp * (e2 / 2), ovf);
                         add_checked(acc, tri, ovf);
                     }
-                } else if (h + 4 == k && near_clique(X, row, mem, lane_above, cx)) {
-                    // h = k-4 over a near-clique X: the subtree holds
+                } else if (h + 4 == k) {
+                    // h = k-4: the subtree holds
                     // C(p,4) + C(p,3)|X| + C(p,2)|E(X)| + p c3(X) + c4(X) k-cliques
                     const CliqueCounts cc = k34_complement(X, row, mem, cx);
                     add_checked(acc, binom_chk(a, p, 4, ovf), ovf);
@@ -1090,13 +1083,27 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
         const uint64_t r1 = lane < R ? rows[lane] : 0ull;
         return sct64(a, k, r1, R >= 64 ? ~0ull : ((1ull << R) - 1), h0, p0, nodes, ovf);
     }
-    uint64_t row[S][S];
+    // rows of the lane in VGPRs for the two-word kernel; the four-word level
+    // (a small share of the work once subtrees of <= 128 candidates are
+    // compacted) reads them from LDS, which keeps its register budget at the
+    // two-word kernel's
+    constexpr bool REG = S <= 2;
+    constexpr int SR = REG ? S : 1;
+    uint64_t row[SR][SR];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < SR; ++s) {
         const uint32_t j = lane + 64u * s;
 #pragma unroll
-        for (int w = 0; w < S; ++w) row[s][w] = (j < R && (uint32_t)w < W) ? rows[(uint64_t)j * W + w] : 0ull;
+        for (int w = 0; w < SR; ++w) row[s][w] = (REG && j < R && (uint32_t)w < W) ? rows[(uint64_t)j * W + w] : 0ull;
     }
+    auto row_word = [&](int s, int w) -> uint64_t {
+        if constexpr (REG) {
+            return row[s][w];
+        } else {
+            const uint32_t j = lane + 64u * s;
+            return (j < R && (uint32_t)w < W) ? rows[(uint64_t)j * W + w] : 0ull;
+        }
+    };
     uint64_t X[S], stkC[S], stkB[S];
     uint32_t stkM = 0;
 #pragma unroll
@@ -1140,7 +1147,7 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                 for (int s = 0; s < S; ++s) {
                     uint32_t v = 0;
 #pragma unroll
-                    for (int w = 0; w < S; ++w) v += (uint32_t)__popcll(X[w] & row[s][w]);
+                    for (int w = 0; w < S; ++w) v += (uint32_t)__popcll(X[w] & row_word(s, w));
                     if ((X[s] >> lane) & 1ull) {
                         lsum += v;
                         const uint32_t kk = ((v + 1) << 16) | (0xFFFFu - (64u * s + lane));
@@ -1311,9 +1318,9 @@ __device__ __attribute__((always_inline)) void finish_wave(const CountArgs &a, W
 }
 
 // register-bitmap pivoting kernel for tasks of at most 64*S vertices
-// waves per SIMD the register budget is held to: 3 for S = 4 (<= 168 VGPRs), 4 for S = 2, 5 for S = 1
+// waves per SIMD the register budget is held to: 4 for S = 4 (<= 128 VGPRs), 6 for S = 2, 8 for S = 1
 template <int S>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S == 4 ? 3 : S == 2 ? 4 : 5)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S == 4 ? 4 : S == 2 ? 6 : 8)))
 void k_pivot_reg(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
@@ -1323,7 +1330,9 @@ void k_pivot_reg(CountArgs a)
     uint32_t *offs = reinterpret_cast<uint32_t *>(lds + L.offs);
     uint32_t *starts = reinterpret_cast<uint32_t *>(lds + L.starts);
     uint64_t *rows = lds + L.rows;
-    uint64_t *crows = lds + L.total;  // (S = 4) the two-word rows of a compacted subtree: 128 x 2 words
+    // (S = 4) the two-word rows of a compacted subtree, 128 x 2 words, over
+    // list and starts (256 u32 each, dead once the bitmap is built)
+    uint64_t *crows = lds + L.list;
     WaveAcc w;
     for (;;) {
         uint32_t start = 0;
@@ -1952,7 +1961,6 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         if (pivot && !global && c <= 2 && !lc) {
             kern = register_kernel(c);
             L = make_layout(a.rmax, a.wmax, 0, 0, true);
-            if (c == 2) L.total += 256;  // the two-word rows of a compacted subtree
             variant[b] = 2u;
         }
         // large tasks of pivoting: blocks of several waves sharing one bitmap
