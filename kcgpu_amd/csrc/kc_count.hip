@@ -445,17 +445,48 @@ __device__ inline CliqueCounts k34_complement(uint64_t X, uint64_t row, bool mem
     const uint64_t Hl = mem ? (X & ~row & ~(1ull << lane)) : 0ull;
     const int32_t d = (int32_t)__popcll(Hl), n = (int32_t)cx;
     uint32_t q = 0, qE = 0, t2 = 0, sd = 0;
-    for (uint64_t bits = X; bits; bits &= bits - 1) {
-        const uint32_t u = (uint32_t)__builtin_ctzll(bits);
-        const uint64_t Hu = X & ~rdlane64(row, u) & ~(1ull << u);
-        const uint32_t du = (uint32_t)__popcll(Hu);
-        const uint32_t c = (uint32_t)__popcll(Hl & Hu);
-        const uint32_t c2 = c * c;
-        q += c2;
-        if ((Hu >> lane) & 1ull) {
-            qE += c2;
-            t2 += c;
-            sd += du;
+    if ((X >> 32) == 0) {
+        // X within vertices 0..31: lanes l and l+32 both hold vertex l (32-bit
+        // rows) and walk two members u per step, one per half of the wave
+        const uint32_t lo = lane & 31, X32 = (uint32_t)X;
+        const bool upper = lane >= 32;
+        const uint32_t row32 = (uint32_t)__shfl((int)(uint32_t)row, (int)lo, 64);
+        const uint32_t H32 = ((X32 >> lo) & 1u) ? (X32 & ~row32 & ~(1u << lo)) : 0u;
+        for (uint32_t bits = X32; bits;) {
+            const uint32_t ua = (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1;
+            uint32_t Hb = 0;
+            if (bits) {
+                const uint32_t ub = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1;
+                Hb = X32 & ~(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)ub) & ~(1u << ub);
+            }
+            const uint32_t Ha = X32 & ~(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)row, (int)ua) & ~(1u << ua);
+            const uint32_t Hu = upper ? Hb : Ha;
+            const uint32_t du = upper ? (uint32_t)__popc(Hb) : (uint32_t)__popc(Ha);
+            const uint32_t c = (uint32_t)__popc(H32 & Hu), c2 = c * c;
+            const uint32_t in = (Hu >> lo) & 1u;  // the lane's vertex is a complement neighbour of u
+            q += c2;
+            qE += in * c2;
+            t2 += in * c;
+            sd += in * du;
+        }
+        q += (uint32_t)__shfl_xor((int)q, 32, 64);
+        qE += (uint32_t)__shfl_xor((int)qE, 32, 64);
+        t2 += (uint32_t)__shfl_xor((int)t2, 32, 64);
+        sd += (uint32_t)__shfl_xor((int)sd, 32, 64);
+        if (upper) q = qE = t2 = sd = 0;  // (Hl, d are 0 there: X has no member above 31)
+    } else {
+        for (uint64_t bits = X; bits; bits &= bits - 1) {
+            const uint32_t u = (uint32_t)__builtin_ctzll(bits);
+            const uint64_t Hu = X & ~rdlane64(row, u) & ~(1ull << u);
+            const uint32_t du = (uint32_t)__popcll(Hu);
+            const uint32_t c = (uint32_t)__popcll(Hl & Hu), c2 = c * c;
+            const uint32_t in = (uint32_t)(Hu >> lane) & 1u;
+            q += c2;
+            qE += in * c2;
+            t2 += in * c;
+            sd += in * du;
         }
     }
     const int32_t t = (int32_t)(t2 / 2);  // H-triangles through the lane's vertex
