@@ -1762,44 +1762,69 @@ __device__ inline int class_of(uint32_t s)
     return s <= 64 ? 0 : s <= 128 ? 1 : s <= 256 ? 2 : s <= 512 ? 3 : s <= 1024 ? 4 : 5;
 }
 
-// Bin tasks by (algorithm, size class); a task smaller than the clique part
-// still to find (k-1 for a node, k-2 for an edge) is dropped.  Pass 0 counts
-// and takes the max size per bin, pass 1 scatters (query << 32 | task).
+// Bin tasks by (algorithm, size class, size slice); a task smaller than the
+// clique part still to find (k-1 for a node, k-2 for an edge) is dropped.
+// Each class is cut into NSUB slices of its size range, largest first, and a
+// class's slices are contiguous in the task array, so a class launch drains
+// its largest tasks first and the long ones do not start last.  Pass 0
+// counts per slice and takes the max size of the unbounded class and of the
+// launch (bin_max[15]), pass 1 scatters
+// (query << 32 | task).  A wave handles the distinct slices among its lanes
+// one at a time (ballot per slice, one atomic per slice).
+constexpr int NSUB = 16;
+
+__device__ inline int slice_of(uint32_t s, int c)
+{
+    if (c == NCLASS - 1) return 0;
+    const uint32_t lo = c == 0 ? 0u : CLASS_ROWS[c - 1], hi = CLASS_ROWS[c];
+    const uint32_t sl = (hi - s) * NSUB / (hi - lo);
+    return sl < NSUB ? (int)sl : NSUB - 1;
+}
+
 __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t *qalgo, const uint64_t *qoff,
-                                                   uint32_t nq, uint64_t total, const uint32_t *size, uint32_t *bin_cnt,
-                                                   uint32_t *bin_max, uint32_t *bin_pos, const uint32_t *bin_off,
+                                                   uint32_t nq, uint64_t total, const uint32_t *size, uint32_t *sub_cnt,
+                                                   uint32_t *bin_max, uint32_t *sub_pos, const uint32_t *sub_off,
                                                    uint64_t *out, int scatter)
 {
     const int lane = threadIdx.x & 63;
     for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; base < total;
          base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t gi = base + lane;
-        int c = -1;
+        int fb = -1;
         uint32_t s = 0, q = 0;
         if (gi < total) {
             q = query_of(qoff, nq, gi);
             const QDesc d = qd[q];
             const uint32_t need = d.gran ? d.k - 2 : d.k - 1;
             s = size[gi];
-            if (s >= need) c = class_of(s) + (qalgo[q] ? NCLASS : 0);
+            if (s >= need) {
+                const int c = class_of(s);
+                fb = ((qalgo[q] ? NCLASS : 0) + c) * NSUB + slice_of(s, c);
+            }
         }
-        for (int b = 0; b < 2 * NCLASS; ++b) {
-            const uint64_t bal = __ballot(c == b);
-            if (!bal) continue;
-            const int leader = __ffsll((unsigned long long)bal) - 1;
+        if (!scatter) {  // the largest task size of the launch (binomial table rows)
+            uint32_t mx = fb >= 0 ? s : 0;
+            for (int dd = 32; dd >= 1; dd >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, dd, 64));
+            if (lane == 0 && mx) atomicMax(&bin_max[15], mx);
+        }
+        for (uint64_t pend = __ballot(fb >= 0); pend;) {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const int cur = __builtin_amdgcn_readlane(fb, leader);
+            const uint64_t bal = __ballot(fb == cur);
+            pend &= ~bal;
             if (!scatter) {
-                uint32_t mx = (c == b) ? s : 0;
-                for (int dd = 32; dd >= 1; dd >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, dd, 64));
-                if (lane == leader) {
-                    atomicAdd(&bin_cnt[b], (uint32_t)__popcll(bal));
-                    atomicMax(&bin_max[b], mx);
+                if (lane == leader) atomicAdd(&sub_cnt[cur], (uint32_t)__popcll(bal));
+                if ((cur / NSUB) % NCLASS == NCLASS - 1) {  // the unbounded class sizes its rows by its max
+                    uint32_t mx = (fb == cur) ? s : 0;
+                    for (int dd = 32; dd >= 1; dd >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, dd, 64));
+                    if (lane == leader) atomicMax(&bin_max[cur / NSUB], mx);
                 }
             } else {
                 uint32_t b0 = 0;
-                if (lane == leader) b0 = atomicAdd(&bin_pos[b], (uint32_t)__popcll(bal));
+                if (lane == leader) b0 = atomicAdd(&sub_pos[cur], (uint32_t)__popcll(bal));
                 b0 = __shfl(b0, leader, 64);
-                if (c == b)
-                    out[bin_off[b] + b0 + (uint32_t)__popcll(bal & lanemask_lt())] =
+                if (fb == cur)
+                    out[sub_off[cur] + b0 + (uint32_t)__popcll(bal & lanemask_lt())] =
                         ((uint64_t)q << 32) | (uint32_t)(gi - qoff[q]);
             }
         }
@@ -1875,7 +1900,9 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     const uint64_t total = hoff[nq];
     if (total >= (1ull << 32) * 64) return fail(KC_ERR_LIMIT, "too many tasks in one launch");
     const size_t qd_bytes = sizeof(QDesc) * nq, off_bytes = sizeof(uint64_t) * (nq + 1);
-    const size_t meta_u32 = 128;  // bin_cnt[16] bin_max[16] bin_pos[16] bin_off[16] qhead[16]
+    constexpr int NS = NB * NSUB;  // size slices
+    // sub_cnt[NS] bin_max[16] sub_pos[NS] sub_off[NS] qhead[16]
+    const size_t meta_u32 = 3 * NS + 32;
     const size_t acc_bytes = sizeof(uint64_t) * NB * ACC;
     const size_t q_bytes = sizeof(uint64_t) * nq + sizeof(uint32_t) * nq;
     if ((rc = scr.qdesc.ensure(qd_bytes + off_bytes + nq + 16)) ||
@@ -1886,7 +1913,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     uint64_t *doff = reinterpret_cast<uint64_t *>(scr.qdesc.as<uint8_t>() + qd_bytes);
     uint8_t *dalgo = scr.qdesc.as<uint8_t>() + qd_bytes + off_bytes;
     uint32_t *m32 = scr.meta.as<uint32_t>();
-    uint32_t *bin_cnt = m32, *bin_max = m32 + 16, *bin_pos = m32 + 32, *bin_off = m32 + 48, *qhead = m32 + 64;
+    uint32_t *sub_cnt = m32, *bin_max = m32 + NS, *sub_pos = bin_max + 16, *sub_off = sub_pos + NS, *qhead = sub_off + NS;
     uint64_t *dacc = reinterpret_cast<uint64_t *>(m32 + meta_u32);
     uint64_t *qcount = dacc + NB * ACC;
     uint32_t *qovf = reinterpret_cast<uint32_t *>(qcount + nq);
@@ -1900,31 +1927,43 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     hipEventCreate(&e_cls);
     hipEventRecord(e_start, s);
     const int gs = (int)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, (uint64_t)ctx->cu * 16));
+    // task sizes of a single query on the graph they were last computed for are reused
+    const bool sizes_cached = nq == 1 && scr.size_uid == qs[0].o->uid && scr.size_gran == qs[0].q.gran &&
+                              scr.size_total == total;
+    scr.size_uid = 0;
     if (total > 0) {
-        hipLaunchKernelGGL(k_task_size, dim3(gs), dim3(256), 0, s, dq, doff, nq, total, scr.size.as<uint32_t>());
+        if (!sizes_cached)
+            hipLaunchKernelGGL(k_task_size, dim3(gs), dim3(256), 0, s, dq, doff, nq, total, scr.size.as<uint32_t>());
         hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
-                           bin_cnt, bin_max, bin_pos, bin_off, scr.tasks.as<uint64_t>(), 0);
+                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 0);
     }
     KC_HIP(hipGetLastError());
-    uint32_t h[32];
+    scr.size_uid = nq == 1 ? qs[0].o->uid : 0;
+    scr.size_gran = qs[0].q.gran;
+    scr.size_total = total;
+    uint32_t h[NS + 16];
     KC_HIP(hipMemcpyAsync(h, m32, sizeof h, hipMemcpyDeviceToHost, s));
     KC_HIP(hipStreamSynchronize(s));
-    uint32_t bcnt[NB], bmax[NB], boff[16] = {0};
+    uint32_t bcnt[NB], bmax[NB], boff[NB], soff[NS];
     uint64_t tot = 0;
     uint32_t maxsize = 0;
     for (int b = 0; b < NB; ++b) {
-        bcnt[b] = h[b];
-        bmax[b] = h[16 + b];
         boff[b] = (uint32_t)tot;
-        tot += h[b];
-        maxsize = std::max(maxsize, h[16 + b]);
+        bcnt[b] = 0;
+        for (int sl = 0; sl < NSUB; ++sl) {
+            soff[b * NSUB + sl] = (uint32_t)tot;
+            bcnt[b] += h[b * NSUB + sl];
+            tot += h[b * NSUB + sl];
+        }
+        bmax[b] = h[NS + b];  // (kept for the unbounded classes only)
     }
+    maxsize = h[NS + 15];
     if (tot >= 0xFFFFFFFFull) return fail(KC_ERR_LIMIT, "more than 2^32 tasks in one launch");
     if ((rc = scr.tasks.ensure(sizeof(uint64_t) * ((size_t)tot + 1)))) return rc;
-    KC_HIP(hipMemcpyAsync(bin_off, boff, sizeof boff, hipMemcpyHostToDevice, s));
+    KC_HIP(hipMemcpyAsync(sub_off, soff, sizeof soff, hipMemcpyHostToDevice, s));
     if (tot > 0)
         hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
-                           bin_cnt, bin_max, bin_pos, bin_off, scr.tasks.as<uint64_t>(), 1);
+                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 1);
     KC_HIP(hipGetLastError());
     hipEventRecord(e_cls, s);
 

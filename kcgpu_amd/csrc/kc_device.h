@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -56,7 +57,21 @@ int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tm
 // scratch of one launch sequence (grow-only)
 struct Scratch {
     DevBuf tasks, meta, binom, gscr, size, qdesc, local;
+    // what `size` holds when it was filled for one query: the oriented graph
+    // (its uid), the granularity and the task count; task sizes (d+(u), or
+    // |N+(u) & N+(v)|) do not depend on k, so the next query on that graph
+    // reuses them
+    uint64_t size_uid = 0;
+    uint32_t size_gran = 0;
+    uint64_t size_total = 0;
 };
+
+// a number no other oriented graph of the process carries
+inline uint64_t next_uid()
+{
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+}
 
 struct QueryRun {
     const kc_oriented *o;
@@ -89,6 +104,7 @@ kc_query tune_query(const kc_oriented *o, const kc_query *q);
 }  // namespace kc
 
 struct kc_oriented {
+    uint64_t uid = kc::next_uid();
     int dev = 0;
     hipStream_t stream = nullptr;  // the device's library stream (not owned)
     uint32_t n = 0;
