@@ -528,7 +528,7 @@ def main():
                 "kernel": f"{names.get(cls_var[dc], '?')} (size class {dc}, tasks of <= {[64, 128, 256, 512, 1024, 'more'][dc]} vertices)",
                 "launches_per_step": int(cls_launch[dc]), "avg_launch_ms": round(avg_ms, 3),
                 "alg_bytes_per_launch": int(avg_bytes),
-                "note": "VALU-bound SCT on a dense core: algorithmic bytes are tiny next to the bitmap work"}
+                "note": "instruction-issue-bound (VALU + SALU) SCT on a dense core: algorithmic HBM bytes are tiny next to the bitmap work"}
 
     cpu = None
     if not args.no_cpu:

@@ -1764,18 +1764,18 @@ __device__ inline int class_of(uint32_t s)
 
 // Bin tasks by (algorithm, size class, size slice); a task smaller than the
 // clique part still to find (k-1 for a node, k-2 for an edge) is dropped.
-// Each class is cut into NSUB slices of its size range, largest first, and a
-// class's slices are contiguous in the task array, so a class launch drains
-// its largest tasks first and the long ones do not start last.  Pass 0
-// counts per slice and takes the max size of the unbounded class and of the
-// launch (bin_max[15]), pass 1 scatters
+// Each class from 65 to 1024 vertices is cut into NSUB slices of its size
+// range, largest first, and a class's slices are contiguous in the task
+// array, so a class launch drains its largest tasks first and the long ones
+// do not start last.  Pass 0 counts per slice and takes the max size of the
+// unbounded class and of the launch (bin_max[15]), pass 1 scatters
 // (query << 32 | task).  A wave handles the distinct slices among its lanes
 // one at a time (ballot per slice, one atomic per slice).
 constexpr int NSUB = 16;
 
 __device__ inline int slice_of(uint32_t s, int c)
 {
-    if (c == NCLASS - 1) return 0;
+    if (c == 0 || c == NCLASS - 1) return 0;  // (the many small tasks: one slice keeps classification cheap)
     const uint32_t lo = c == 0 ? 0u : CLASS_ROWS[c - 1], hi = CLASS_ROWS[c];
     const uint32_t sl = (hi - s) * NSUB / (hi - lo);
     return sl < NSUB ? (int)sl : NSUB - 1;
