@@ -1053,6 +1053,134 @@ __device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (
     return comp ? cx * (cx - 1) * (cx - 2) / 6 - tot / 2 : tot / 2;
 }
 
+// k34_complement for X of up to 64*S vertices (rows of stride W in LDS): lane
+// l takes the vertices l + 64s, s < S, with their complement rows in S words
+// each; the same uniform walk over u in X, the same four running sums per
+// vertex and the same K4 walk over the vertices in three or more complement
+// triangles, with the lane shares summed in 64 bits (C(256,4) * 24 does not
+// fit 32).  Used at h = k-4 when X is a near-clique (the complement walks
+// stay short).
+template <int S>
+__device__ CliqueCounts k34_complement_rows(const uint64_t *rows, uint32_t W, const uint64_t (&X)[S], uint32_t cx)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const int64_t n = cx;
+    uint64_t Hl[S][S];
+    int64_t d[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t v = lane + 64u * s;
+        const bool mem = (X[s] >> lane) & 1ull;
+        d[s] = 0;
+#pragma unroll
+        for (int w = 0; w < S; ++w) {
+            const uint64_t rw = (mem && (uint32_t)w < W) ? rows[(uint64_t)v * W + w] : ~0ull;
+            Hl[s][w] = X[w] & ~rw & (w == s ? ~(1ull << lane) : ~0ull);
+            d[s] += __popcll(Hl[s][w]);
+        }
+    }
+    uint32_t q[S], qE[S], t2[S], sd[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) q[s] = qE[s] = t2[s] = sd[s] = 0;
+#pragma unroll
+    for (int wu = 0; wu < S; ++wu) {
+        for (uint64_t bits = X[wu]; bits; bits &= bits - 1) {
+            const uint32_t u = 64u * wu + (uint32_t)__builtin_ctzll(bits);
+            uint64_t Hu[S];
+            uint32_t du = 0;
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                Hu[w] = X[w] & ~lds_row_word(rows, W, u, w) & (w == wu ? ~(1ull << (u & 63)) : ~0ull);
+                du += (uint32_t)__popcll(Hu[w]);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                uint32_t c = 0;
+#pragma unroll
+                for (int w = 0; w < S; ++w) c += (uint32_t)__popcll(Hl[s][w] & Hu[w]);
+                const uint32_t c2 = c * c, in = (uint32_t)(Hu[s] >> lane) & 1u;
+                q[s] += c2;
+                qE[s] += in * c2;
+                t2[s] += in * c;
+                sd[s] += in * du;
+            }
+        }
+    }
+    // K4 of H among the vertices in >= 3 H-triangles (bit l of word s = vertex l + 64s)
+    uint64_t T3[S];
+    uint32_t nt3 = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        T3[s] = __ballot(t2[s] >= 6);
+        nt3 += (uint32_t)__popcll(T3[s]);
+    }
+    int64_t k4acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) k4acc[s] = 0;
+    if (nt3 >= 4) {
+#pragma unroll
+        for (int wu = 0; wu < S; ++wu) {
+            for (uint64_t ub = T3[wu]; ub; ub &= ub - 1) {
+                const uint32_t u = 64u * wu + (uint32_t)__builtin_ctzll(ub);
+                uint64_t Hu[S];
+#pragma unroll
+                for (int w = 0; w < S; ++w) {
+                    const uint64_t above = w > wu ? ~0ull : w < wu ? 0ull : ((u & 63) == 63 ? 0ull : (~0ull << ((u & 63) + 1)));
+                    Hu[w] = T3[w] & ~lds_row_word(rows, W, u, w) & above;
+                }
+#pragma unroll
+                for (int wv = 0; wv < S; ++wv) {
+                    for (uint64_t vb = Hu[wv]; vb; vb &= vb - 1) {
+                        const uint32_t v = 64u * wv + (uint32_t)__builtin_ctzll(vb);
+                        uint64_t Sv[S];
+                        uint32_t ns = 0;
+#pragma unroll
+                        for (int w = 0; w < S; ++w) {
+                            const uint64_t above = w > wv ? ~0ull : w < wv ? 0ull : ((v & 63) == 63 ? 0ull : (~0ull << ((v & 63) + 1)));
+                            Sv[w] = Hu[w] & ~lds_row_word(rows, W, v, w) & above;
+                            ns += (uint32_t)__popcll(Sv[w]);
+                        }
+                        if (ns < 2) continue;
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            if (!((Sv[s] >> lane) & 1ull)) continue;
+                            uint32_t z = 0;
+#pragma unroll
+                            for (int w = 0; w < S; ++w) {
+                                const uint64_t above = w > s ? ~0ull : w < s ? 0ull : (lane == 63 ? 0ull : (~0ull << (lane + 1)));
+                                z += (uint32_t)__popcll(Sv[w] & Hl[s][w] & above);
+                            }
+                            k4acc[s] += z;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    int64_t c4l = 0, c3l = 0, sdl = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int64_t dd = d[s], t = t2[s] / 2, iq = q[s], iqE = qE[s], isd = sd[s], it2 = t2[s];
+        c4l += 12 * (n - 4) * dd * (dd - 1) - 8 * (n - 3) * t - 12 * (dd - 1) * (isd - dd) + 24 * t -
+               4 * dd * (dd - 1) * (dd - 2) + 3 * ((iq - dd * dd) - (isd - dd)) + 24 * t * (dd - 2) - 6 * (iqE - it2) +
+               24 * k4acc[s];
+        c3l += -3 * dd * (n - 1 - dd) - 2 * t;
+        sdl += dd;
+    }
+    uint32_t dummy = 0;
+    const int64_t s_d = (int64_t)uni64(wave_sum64_checked((uint64_t)sdl, dummy));
+    const int64_t s_4 = (int64_t)uni64(wave_sum64_checked((uint64_t)c4l, dummy));
+    const int64_t s_3 = (int64_t)uni64(wave_sum64_checked((uint64_t)c3l, dummy));
+    const int64_t m = s_d / 2;
+    const int64_t c4_24 = n * (n - 1) * (n - 2) * (n - 3) - 12 * m * (n - 2) * (n - 3) + 12 * m * (m - 1) + s_4;
+    const int64_t c3_6 = n * (n - 1) * (n - 2) + s_3;
+    CliqueCounts r;
+    r.e = (uint32_t)(n * (n - 1) / 2 - m);
+    r.c3 = (uint32_t)(c3_6 / 6);
+    r.c4 = (uint32_t)(c4_24 / 24);
+    return r;
+}
+
 // Compaction to two words: once a node of the four-word kernel has 65..128
 // candidates, its subtree is re-encoded as a subproblem of at most 128
 // vertices (vertex j = the j-th member of X) with two-word rows in crows and
@@ -1185,9 +1313,18 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                         key = kk > key ? kk : key;
                     }
                 }
-                if (h + 3 >= k) {
-                    // twice the edges of G[X]
-                    const uint32_t e2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(lsum));
+                uint32_t e2 = 0;
+                if (h + 4 >= k) e2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(lsum));
+                if (S == 2 && h + 4 == k && 4 * (uint64_t)e2 >= 3ull * cx * (cx - 1)) {
+                    // h = k-4 over a near-clique (complement density <= 1/4): closed form
+                    const CliqueCounts cc = k34_complement_rows<S>(rows, W, X, cx);
+                    add_checked(acc, binom_chk(a, p, 4, ovf), ovf);
+                    add_checked(acc, mul_chk(binom_chk(a, p, 3, ovf), cx, ovf), ovf);
+                    add_checked(acc, mul_chk(binom_chk(a, p, 2, ovf), cc.e, ovf), ovf);
+                    add_checked(acc, mul_chk(p, cc.c3, ovf), ovf);
+                    add_checked(acc, cc.c4, ovf);
+                } else if (h + 3 >= k) {
+                    // (e2 = twice the edges of G[X])
                     const uint64_t pp = p;
                     if (h + 2 == k) {
                         add_checked(acc, pp * (pp - (pp ? 1 : 0)) / 2, ovf);
