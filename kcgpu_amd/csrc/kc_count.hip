@@ -1710,7 +1710,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
     const WideLayout L = make_wide_layout(a.rmax, a.wmax, a.levels, NW, a.chain);
     uint32_t *ctl = reinterpret_cast<uint32_t *>(ws + L.ctl);  // task, R, next branch, branches, pivot
     uint32_t *list = reinterpret_cast<uint32_t *>(ws + L.list);
-    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the entries (j << 26 | b)
+    uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the entries (j << 16 | b)
     uint32_t *starts = reinterpret_cast<uint32_t *>(ws + L.starts);
     uint64_t *rows = ws + L.rows;
     uint64_t *stk = ws + L.wave + (uint64_t)wv * L.per_wave;
@@ -1807,13 +1807,13 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
                         Bj[x] = bw;
                         Xn[x] = Xj[x] & ~bw & rp[x];
                     }
-                    if ((bw >> lane) & 1ull) offs[ne + (uint32_t)__popcll(bw & lanemask_lt())] = (j << 26) | (x * 64 + lane);
+                    if ((bw >> lane) & 1ull) offs[ne + (uint32_t)__popcll(bw & lanemask_lt())] = (j << 16) | (x * 64 + lane);
                     ne += (uint32_t)__popcll(bw);
                 }
                 sync_tree<true>();
             }
             if (lane == 0) {
-                if (endj != INF32) offs[ne++] = (endj << 26) | 0x3FFFFFFu;
+                if (endj != INF32) offs[ne++] = (endj << 16) | 0xFFFFu;
                 ctl[2] = 0;
                 ctl[3] = ne;
             }
@@ -1825,9 +1825,9 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
             if (lane == 0) i = atomicAdd(&ctl[2], 1u);
             i = uni32(i);
             if (i >= ne) break;
-            const uint32_t e = offs[i], j = e >> 26, b = e & 0x3FFFFFFu;
+            const uint32_t e = offs[i], j = e >> 16, b = e & 0xFFFFu;
             const uint64_t *Xj = chX + (uint64_t)j * 2 * a.wmax, *Bj = chB + (uint64_t)j * 2 * a.wmax;
-            const bool end = (b == 0x3FFFFFFu);
+            const bool end = (b == 0xFFFFu);
             const uint64_t *rb = rows + (uint64_t)(end ? 0 : b) * W;
             for (uint32_t x = lane; x < W; x += 64) {
                 if (end) {
@@ -2174,7 +2174,16 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         uint32_t nw = 1;
         WideLayout WL{};
         if (pivot && !lc && c >= 3) {
-            a.chain = c == 3 ? 32 : 16;
+            // chain levels walked by wave 0 before the entries are shared: the
+            // whole chain for the HBM classes (a chain left whole is one entry,
+            // which one wave would walk alone), 32 levels when LDS holds them
+            if (a.rmax >= 0xFFFFu) {
+                for (auto &e : ev) hipEventDestroy(e);
+                hipEventDestroy(e_start);
+                hipEventDestroy(e_cls);
+                return fail(KC_ERR_LIMIT, "a task of %u vertices exceeds the wide kernel's 65534", a.rmax);
+            }
+            a.chain = c == 3 ? 32u : std::min<uint32_t>(1024u, std::max<uint32_t>(64u, a.rmax));
             bool fits = false;
             if (c < NCLASS - 1)
                 for (uint32_t t = 16; t >= 2 && !fits; t >>= 1) {
