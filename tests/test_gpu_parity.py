@@ -127,6 +127,40 @@ def test_size_classes_vs_oracle(mode):
                 assert r.count == exp, (mode, k, algo, gran)
 
 
+def _planted_near(n, m, specs, seed):
+    """sparse ER with planted near-cliques (size, edge density): dense candidate
+    sets of 65..256 vertices whose complements hold triangles and 4-cliques"""
+    rng = np.random.default_rng(seed)
+    G0 = K.Graph.generate(K.GEN_ER, seed=seed, n=n, m=m)
+    rp, col = G0.csr()
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    es, ed = [src], [col]
+    for s, dens in specs:
+        vs = rng.choice(n, size=s, replace=False).astype(np.uint32)
+        a, b = np.triu_indices(s, 1)
+        keep = rng.random(a.size) < dens
+        es.append(vs[a[keep]])
+        ed.append(vs[b[keep]])
+    return K.Graph.from_edges(np.concatenate(es), np.concatenate(ed), n)
+
+
+def test_near_clique_closures_vs_oracle():
+    # the h = k-4 complement closures (one-word and two-word rows), the
+    # two-word compaction and the triangle walks all run on these subproblems
+    G = _planted_near(3000, 12000, [(90, 0.92), (140, 0.85), (200, 0.9)], seed=7)
+    rp, col = _host_csr(G)
+    pr, _ = O.peel_priority(rp, col)
+    orp, oc = O.orient(rp, col, pr)
+    for k in (5, 6, 7):
+        exp = O.count(orp, oc, k, "pivot_tails", "edge")["count"]
+        for mode in ("degree", "degen"):
+            Og = G.orient(mode)
+            for gran in GRANS:
+                r = Og.count(k, "pivot", gran, part_size=1)
+                print(mode, k, gran, r.count, r.raw["count_ms"], flush=True)
+                assert r.count == exp, (mode, k, gran, r.count, exp)
+
+
 @pytest.mark.parametrize("kind,kw", [
     (K.GEN_ER, dict(n=10000, m=100000)),  # config 1 shape
     (K.GEN_BA, dict(n=20000, attach=12)),
