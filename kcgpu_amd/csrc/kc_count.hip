@@ -710,7 +710,9 @@ __device__ inline void sync_tree()
 template <int S>
 __device__ uint32_t tri_lanes(const uint64_t *rows, uint32_t W, const uint64_t (&X)[S], uint32_t cx, uint32_t e2,
                               uint32_t *idx);
-template <int S>
+// CL4: the four-word kernel also closes h = k-4 near-cliques (the wide kernel
+// has the registers for it; the one-wave four-word kernel keeps its budget)
+template <int S, bool CL4 = false>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
                               uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows);
 
@@ -729,7 +731,7 @@ template <bool LOCAL, bool WIDE = false>
 __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
                                 Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
                                 uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true,
-                                uint64_t *crows2 = nullptr)
+                                uint64_t *crows2 = nullptr, uint64_t *crows4 = nullptr)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -802,6 +804,28 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                 }
                 add_checked(acc, sct64(a, k, row1, cx >= 64 ? ~0ull : ((1ull << cx) - 1), h, p, nodes, ovf), ovf);
                 --my_nodes;  // counted again as the subtree's root
+            } else if (!LOCAL && crows4 && cx > 128 && cx <= 256 && h + 3 <= k && h + p + cx >= k) {
+                // 129..256 candidates: re-encode three or four words per row the
+                // same way and finish the subtree in the four-word register SCT
+                // (which closes an h = k-4 near-clique at its root)
+                const uint32_t Wc = (cx + 63) / 64;
+                uint32_t mt[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) mt[t] = 64u * t + lane < cx ? cand[64u * t + lane] : 0u;
+                for (uint32_t j = 0; j < cx; ++j) {
+                    const uint32_t xj = cand[j];
+                    uint64_t mine = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint64_t wv = 64u * t + lane < cx ? rows[(uint64_t)mt[t] * W + (xj >> 6)] : 0ull;
+                        const uint64_t col = __ballot((wv >> (xj & 63)) & 1ull);
+                        mine = lane == (uint32_t)t ? col : mine;
+                    }
+                    if (lane < Wc) crows4[(uint64_t)Wc * j + lane] = mine;
+                }
+                sync_tree<WIDE>();
+                add_checked(acc, pivot_reg<4, true>(a, k, crows4, cx, h, cand, nodes, ovf, p, crows2), ovf);
+                --my_nodes;
             } else if (!LOCAL && crows2 && cx <= 128 && h + 3 <= k && h + p + cx >= k) {
                 // 65..128 candidates: re-encode two words per row (word t of new
                 // row j = ballot of bit cand[j] over the rows of candidates
@@ -1188,7 +1212,7 @@ __device__ CliqueCounts k34_complement_rows(const uint64_t *rows, uint32_t W, co
 // quarter of the four-word ones.  Lanes l and l+64 load the full rows of
 // members l and l+64; word t of new row j is the ballot over lanes of bit x_j
 // of the rows of members 64t + lane (the bitmap is symmetric).
-template <int S>
+template <int S, bool CL4>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
                               uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows);
 
@@ -1228,7 +1252,7 @@ __device__ uint64_t compact2_and_count(const CountArgs &a, uint32_t k, const uin
     return pivot_reg<2>(a, k, crows, cx, h, idx, nodes, ovf, p, nullptr);
 }
 
-template <int S>
+template <int S, bool CL4>
 __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *rows, uint32_t R, uint32_t h0,
                               uint32_t *idx, uint64_t &nodes, uint32_t &ovf, uint32_t p0, uint64_t *crows)
 {
@@ -1315,7 +1339,7 @@ __device__ uint64_t pivot_reg(const CountArgs &a, uint32_t k, const uint64_t *ro
                 }
                 uint32_t e2 = 0;
                 if (h + 4 >= k) e2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__ockl_wfred_add_u32(lsum));
-                if (S == 2 && h + 4 == k && 4 * (uint64_t)e2 >= 3ull * cx * (cx - 1)) {
+                if ((S == 2 || (S == 4 && CL4)) && h + 4 == k && 4 * (uint64_t)e2 >= 3ull * cx * (cx - 1)) {
                     // h = k-4 over a near-clique (complement density <= 1/4): closed form
                     const CliqueCounts cc = k34_complement_rows<S>(rows, W, X, cx);
                     add_checked(acc, binom_chk(a, p, 4, ovf), ovf);
@@ -1654,7 +1678,8 @@ __host__ __device__ inline WideLayout make_wide_layout(uint32_t rmax, uint32_t w
     L.chain = L.starts + (rmax + 1) / 2;         // per chain level: candidates, non-pivot branches
     L.rows = L.chain + (uint64_t)chain * 2 * wmax;
     L.wave = L.rows + (uint64_t)rmax * wmax;
-    L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2 + 256;  // + two-word rows
+    // + the rows of compacted subtrees: 128 x 2 and 256 x 4 words
+    L.per_wave = (uint64_t)levels * 2 * wmax + wmax + (uint64_t)levels * 2 + (rmax + 1) / 2 + 256 + 1024;
     L.total = L.wave + L.per_wave * nw;
     return L;
 }
@@ -1702,7 +1727,7 @@ __device__ void build_rows_block(const Cur &g, const uint32_t *list, uint32_t *o
 }
 
 template <bool GLOBAL>
-__global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
+__global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
 {
     extern __shared__ uint64_t lds[];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, NW = blockDim.x >> 6;
@@ -1718,6 +1743,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
     Scal *scal = reinterpret_cast<Scal *>(X + a.wmax);
     uint32_t *cand = reinterpret_cast<uint32_t *>(X + a.wmax + (uint64_t)a.levels * 2);
     uint64_t *crows2 = X + a.wmax + (uint64_t)a.levels * 2 + (a.rmax + 1) / 2;  // 128 x 2 words
+    uint64_t *crows4 = crows2 + 256;                                           // 256 x 4 words
     WaveAcc w;
     uint64_t no_bytes = 0;
     for (;;) {
@@ -1841,7 +1867,7 @@ __global__ __launch_bounds__(1024) void k_pivot_wide(CountArgs a)
             const uint32_t h = end ? h0 : h0 + 1;
             add_checked(w.run,
                         pivot_count<false, true>(a, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf, nullptr,
-                                                 j, false, crows2),
+                                                 j, false, crows2, crows4),
                         w.run_ovf);
         }
         __syncthreads();
@@ -2186,7 +2212,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             a.chain = c == 3 ? 32u : std::min<uint32_t>(1024u, std::max<uint32_t>(64u, a.rmax));
             bool fits = false;
             if (c < NCLASS - 1)
-                for (uint32_t t = 16; t >= 2 && !fits; t >>= 1) {
+                for (uint32_t t = 8; t >= 2 && !fits; t >>= 1) {
                     WL = make_wide_layout(a.rmax, a.wmax, a.levels, t, a.chain);
                     if (WL.total * 8 <= lds_cap) {
                         nw = t;
