@@ -870,6 +870,37 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
                     add_checked(acc, (uint64_t)p * cx, ovf);
                     add_checked(acc, tot_s / 2, ovf);
                     if (LOCAL) credit_hp(acc - before, (p ? (uint64_t)p - 1 : 0) + cx);
+                } else if (LOCAL && h + 3 == k) {
+                    // h = k-3 with per-vertex credits: x in X lies in
+                    // C(p,2) + p deg_X(x) + t_X(x) of the subtree's cliques (t_X(x) =
+                    // edges of G[X] inside N(x), walked by the lane that owns x),
+                    // a pivot in C(p-1,2) + (p-1)|X| + |E(X)|, a held vertex in all
+                    sync_tree<WIDE>();  // (lacc of the score pass)
+                    const uint64_t pp = p, cp2 = pp * (pp - (pp ? 1 : 0)) / 2;
+                    uint64_t t3 = 0;
+                    for (uint32_t c = lane; c < cx; c += 64) {
+                        const uint32_t x = cand[c];
+                        const uint64_t *rx = rows + (uint64_t)x * W;
+                        uint64_t deg = 0, t2 = 0;
+                        for (uint32_t w = 0; w < W; ++w) {
+                            const uint64_t yw = X[w] & rx[w];
+                            deg += (uint64_t)__popcll(yw);
+                            for (uint64_t bits = yw; bits; bits &= bits - 1) {
+                                const uint64_t *ry = rows + (uint64_t)(w * 64 + (uint32_t)__builtin_ctzll(bits)) * W;
+                                for (uint32_t v = 0; v < W; ++v) t2 += (uint64_t)__popcll(X[v] & rx[v] & ry[v]);
+                            }
+                        }
+                        lacc[x] += cp2 + pp * deg + t2 / 2;
+                        t3 += t2 / 2;
+                    }
+                    uint32_t dummy = 0;
+                    const uint64_t tri = wave_sum64_checked(t3, dummy) / 3;
+                    const uint64_t e = tot_s / 2;
+                    const uint64_t before = acc;
+                    add_checked(acc, close_k3(a, p, cx, e, tri, ovf), ovf);
+                    const uint64_t q = pp ? pp - 1 : 0;
+                    credit_hp(acc - before, p ? q * (q - (q ? 1 : 0)) / 2 + q * cx + e : 0);
+                    sync_tree<WIDE>();
                 } else if (!LOCAL && h + 3 == k && W <= 16) {
                     // h = k-3: closed form with the triangles of G[X] (lane walks)
                     const uint32_t e2 = (uint32_t)tot_s;
