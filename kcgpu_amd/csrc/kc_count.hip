@@ -132,13 +132,6 @@ __device__ inline uint32_t wave_incl_scan32(uint32_t x)
     return x;
 }
 
-__device__ inline uint32_t wave_sum32(uint32_t x)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
-    return x;
-}
-
 __device__ inline uint32_t wave_min32(uint32_t x)
 {
 #pragma unroll
@@ -669,8 +662,10 @@ struct Scal {
 // With c_0 = 1, c_1 = |X| and c_2 = |E(X)| = (1/2) sum_{j in X} |X & row_j|
 // (the pivot scores) the subtree is closed-form once h >= k-2:
 //     h = k:   1        h = k-1:   p + |X|        h = k-2:  C(p,2) + p|X| + |E(X)|
-// so the SCT is only expanded while h < k-2.  The register-bitmap kernel also
-// closes h = k-3 with c_3 = triangles of G[X].  A node is also dropped when
+// so the SCT is only expanded while h < k-2.  h = k-3 closes with c_3 =
+// triangles of G[X] (lane walks; with per-vertex credits for LOCAL), and the
+// register kernels close h = k-4 with c_3 and c_4 of near-cliques from the
+// complement.  A node is also dropped when
 // h + p + max_j|X & row_j| + 1 < k (the largest clique inside X has at most
 // maxIntersection + 1 vertices, cf. kclique/kckernels.cuh:2573-2584).
 //
@@ -968,21 +963,13 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *
 }
 
 // ---- step 3c: pivoting with the bitmap in registers (R <= 64*S) -------------
-// Lane l holds rows l, l+64, ..., l+64(S-1) of the full bitmap in VGPRs, so a
-// pivot score is S*S AND+popcount on the lane's own registers; the rows a
-// branch needs whole are broadcast-read from the LDS copy left by the build.  The SCT
+// Lane l holds rows l, l+64, ..., l+64(S-1) of the bitmap (in VGPRs for the
+// one- and two-word kernels; the four-word level reads them from LDS), so a
+// pivot score is S*S AND+popcount on the lane's own rows; the rows a branch
+// needs whole are broadcast-read from the LDS copy left by the build.  The SCT
 // state is uniform: candidate set X and the level variables live in scalar
 // registers, and the stack is spread across the lanes (lane t keeps level t in
-// VGPRs, read back with v_readlane).  Arg-max and sums use ballots: the max
-// by a bitwise search, the sum bit-sliced, so no LDS and no shuffles are
-// touched after the rows are loaded.
-
-__device__ inline uint32_t ballot_sum(uint32_t v, int bits)
-{
-    uint32_t s = 0;
-    for (int b = 0; b < bits; ++b) s += (uint32_t)__popcll(__ballot((v >> b) & 1u)) << b;
-    return s;
-}
+// VGPRs, read back with v_readlane).  Arg-max and sums are wave reductions.
 
 // word w of row b from the LDS copy (uniform address: a broadcast read)
 __device__ inline uint64_t lds_row_word(const uint64_t *rows, uint32_t W, uint32_t b, int w)
