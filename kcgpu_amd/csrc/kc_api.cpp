@@ -134,7 +134,6 @@ void kc_graph_free(kc_graph *g) { delete g; }
 int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan)
 {
     if (!g || !plan) return fail(KC_ERR_ARG, "kc_autotune: null argument");
-    (void)k;
     const HostCSR &c = g->csr;
     uint32_t maxout = 0;
 #pragma omp parallel for reduction(max : maxout) schedule(static, 4096)
@@ -148,11 +147,18 @@ int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan)
         maxout = out > maxout ? out : maxout;
     }
     plan->degree_max_out = maxout;
-    plan->orient_mode = maxout <= 64 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
     plan->algo = KC_ALGO_PIVOT;
-    plan->gran = maxout <= 64 ? KC_GRAN_NODE : KC_GRAN_EDGE;
     plan->part_size = 1;
-    return KC_OK;
+    if (maxout <= 64 || k < 4) {
+        // every task fits one word in the degree order: no peel, node tasks
+        plan->orient_mode = maxout <= 64 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
+        plan->gran = maxout <= 64 ? KC_GRAN_NODE : KC_GRAN_EDGE;
+        return KC_OK;
+    }
+    // larger tasks: which orientation and granularity is cheaper depends on
+    // the graph (measured: degree order and edge tasks on a Chung-Lu graph,
+    // degeneracy order and node tasks on R-MAT), so it is measured
+    return kc::autotune_measured(g, k, maxout, plan);
 }
 
 // -q grammar (include/Config.h:235-271): [o|p]<P>[b|n|h]

@@ -46,7 +46,7 @@ constexpr int ACC = 8;  // per-class accumulator slots
 // one query of a launch: the oriented graph it runs on and its k
 struct QDesc {
     const uint32_t *rowptr, *col, *src;
-    uint32_t k, gran, pad0, pad1;
+    uint32_t k, gran, sample, pad;
 };
 
 // the graph and k of the task a wave is working on
@@ -1978,7 +1978,7 @@ __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t
             const QDesc d = qd[q];
             const uint32_t need = d.gran ? d.k - 2 : d.k - 1;
             s = size[gi];
-            if (s >= need) {
+            if (s >= need && (d.sample <= 1 || (uint32_t)(gi - qoff[q]) % d.sample == 0)) {
                 const int c = class_of(s);
                 fb = ((qalgo[q] ? NCLASS : 0) + c) * NSUB + slice_of(s, c);
             }
@@ -2072,7 +2072,8 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     for (uint32_t i = 0; i < nq; ++i) {
         const kc_oriented *o = qs[i].o;
         const kc_query &q = qs[i].q;
-        hq[i] = QDesc{o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(), o->src.as<uint32_t>(), q.k, q.gran, 0, 0};
+        hq[i] = QDesc{o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(), o->src.as<uint32_t>(), q.k, q.gran,
+                      std::max<uint32_t>(1, qs[i].sample), 0};
         halgo[i] = (uint8_t)(q.algo == KC_ALGO_PIVOT);
         hoff[i + 1] = hoff[i] + (q.gran == KC_GRAN_EDGE ? o->m : o->n);
         kmax = std::max(kmax, q.k);
@@ -2391,6 +2392,52 @@ static void fill_result(kc_result *r, const kc_result &res, const LaunchStats &l
         r->class_max_task_ms[c] = ls.class_max_task_ms[c];
         r->class_task_ms_sum[c] = ls.class_task_ms_sum[c];
     }
+}
+
+// ---- measured autotuner ---------------------------------------------------
+// Both orientations are built and each (orientation, granularity) pair counts
+// every 64th task at this k with pivoting; the pair with the least device
+// time (classification + count of the sample) is the plan.  (Summed task
+// times, tail-free, ranked the plans worse: a sample of few large tasks runs
+// at another occupancy than one of many small ones.)
+int kc::autotune_measured(const kc_graph *g, uint32_t k, uint32_t degree_max_out, kc_plan *plan)
+{
+    constexpr uint32_t SAMPLE = 64;
+    kc_oriented *o[2] = {nullptr, nullptr};
+    int rc = kc_orient(g, KC_ORIENT_DEGREE, &o[0]);
+    if (rc == KC_OK) rc = kc_orient(g, KC_ORIENT_DEGEN, &o[1]);
+    double best = 0;
+    int best_o = KC_ORIENT_DEGEN;
+    uint32_t best_g = KC_GRAN_EDGE;
+    bool have = false;
+    for (int oi = 0; oi < 2 && rc == KC_OK; ++oi) {
+        for (uint32_t gran = KC_GRAN_NODE; gran <= KC_GRAN_EDGE && rc == KC_OK; ++gran) {
+            QueryRun r;
+            r.o = o[oi];
+            r.q = kc_query{k, KC_ALGO_PIVOT, gran, 1, 1, 0};
+            r.sample = SAMPLE;
+            std::vector<kc_result> res;
+            LaunchStats ls;
+            hipSetDevice(o[oi]->dev);
+            rc = run_queries(o[oi]->scr, o[oi]->stream, {r}, res, &ls);
+            const double t = ls.count_ms + ls.prep_ms;
+            if (rc == KC_OK && (!have || t < best)) {
+                have = true;
+                best = t;
+                best_o = oi == 0 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
+                best_g = gran;
+            }
+        }
+    }
+    for (kc_oriented *x : o)
+        if (x) kc_oriented_free(x);
+    if (rc != KC_OK) return rc;
+    plan->degree_max_out = degree_max_out;
+    plan->orient_mode = best_o;
+    plan->algo = KC_ALGO_PIVOT;
+    plan->gran = best_g;
+    plan->part_size = 1;
+    return KC_OK;
 }
 
 extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)

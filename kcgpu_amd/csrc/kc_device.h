@@ -76,6 +76,7 @@ inline uint64_t next_uid()
 struct QueryRun {
     const kc_oriented *o;
     kc_query q;
+    uint32_t sample = 1;  // count only the tasks whose id is a multiple of this (the measured autotuner)
 };
 
 struct LaunchStats {

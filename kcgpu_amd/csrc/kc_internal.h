@@ -32,6 +32,10 @@ uint64_t fnv1a_csr(const HostCSR &g);
 int load_file(const char *path, HostCSR &out);
 int read_edges_reference(const char *path, std::vector<uint32_t> &src, std::vector<uint32_t> &dst);
 
+// Measured plan (kc_count.hip): both orientations, each granularity counted on
+// a sample of its tasks at this k; the fastest wins.
+int autotune_measured(const kc_graph *g, uint32_t k, uint32_t degree_max_out, kc_plan *plan);
+
 // Generators (kc_gen.cpp): edge lists (u, v) with u != v.
 int generate(const kc_gen_params &p, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
              uint32_t &n);

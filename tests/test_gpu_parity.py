@@ -161,6 +161,19 @@ def test_near_clique_closures_vs_oracle():
                 assert r.count == exp, (mode, k, gran, r.count, exp)
 
 
+def test_measured_autotune_plan_counts():
+    # a graph with out-degrees above 64: the plan is measured; counting with
+    # it gives the oracle's count
+    G = K.Graph.generate(K.GEN_CHUNG_LU, seed=1, n=100000, m=600000, gamma=2.1, max_degree=5000.0)
+    rp, col = _host_csr(G)
+    orp, oc = O.orient(rp, col)
+    for k in (5, 6):
+        plan = G.autotune(k)
+        assert plan["algo"] == "pivot" and plan["orient"] in ("degree", "degen") and plan["gran"] in GRANS
+        r = G.orient(plan["orient"]).count(k, "pivot", plan["gran"], part_size=plan["part_size"])
+        assert r.count == O.count(orp, oc, k, "pivot_tails", "edge")["count"], (k, plan)
+
+
 @pytest.mark.parametrize("kind,kw", [
     (K.GEN_ER, dict(n=10000, m=100000)),  # config 1 shape
     (K.GEN_BA, dict(n=20000, attach=12)),

@@ -229,13 +229,19 @@ def test_device_calls_fail_cleanly_without_gpu():
 def test_autotune_rules():
     ba = K.Graph.generate(K.GEN_BA, seed=1, n=20000, attach=20).autotune(6)
     assert ba["orient"] == "degree" and ba["gran"] == "node" and ba["degree_max_out"] <= 64
-    cl = K.Graph.generate(K.GEN_CHUNG_LU, seed=1, n=100000, m=600000, gamma=2.1, max_degree=5000.0).autotune(6)
+    clg = K.Graph.generate(K.GEN_CHUNG_LU, seed=1, n=100000, m=600000, gamma=2.1, max_degree=5000.0)
+    cl = clg.autotune(3)
     assert cl["orient"] == "degen" and cl["gran"] == "edge" and cl["degree_max_out"] > 64
     # the degree-order out-degree the plan reports is the oracle's
     G = K.Graph.generate(K.GEN_CHUNG_LU, seed=3, n=20000, m=80000, gamma=2.1, max_degree=900.0)
     rp, col = G.csr()
     orp, _ = O.orient(rp.astype(np.uint64), col)
-    assert G.autotune(5)["degree_max_out"] == int(np.diff(orp).max())
+    assert G.autotune(3)["degree_max_out"] == int(np.diff(orp).max())
+    # above 64 out-neighbours at k >= 4 the plan is measured on the device:
+    # without one the call fails loudly
+    with pytest.raises(K.KCError) as ei:
+        clg.autotune(6)
+    assert ei.value.code == K.KC_ERR_DEVICE
 
 
 def test_cli_usage_and_argument_errors(tmp_path):
