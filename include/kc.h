@@ -264,8 +264,11 @@ typedef struct {
 
 /*
  * Pick orientation, granularity, algorithm and partition size for counting
- * k-cliques of g (the north-star autotuner; the reference takes all four
- * from the command line).  One O(m) host pass over the CSR.
+ * k-cliques of g (the reference takes all four from the command line).  One
+ * O(m) host pass over the CSR; when the degree order leaves out-degrees
+ * above 64 and k >= 4, the orientation and granularity are measured on the
+ * device (both orientations built, a 1/64 sample of the tasks counted per
+ * pair), which needs a GPU and costs up to about one count.
  */
 int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan);
 
