@@ -2403,6 +2403,7 @@ static void fill_result(kc_result *r, const kc_result &res, const LaunchStats &l
 int kc::autotune_measured(const kc_graph *g, uint32_t k, uint32_t degree_max_out, kc_plan *plan)
 {
     constexpr uint32_t SAMPLE = 64;
+    if (k > KC_MAX_K) return fail(KC_ERR_ARG, "k=%u outside 1..%d", k, KC_MAX_K);
     kc_oriented *o[2] = {nullptr, nullptr};
     int rc = kc_orient(g, KC_ORIENT_DEGREE, &o[0]);
     if (rc == KC_OK) rc = kc_orient(g, KC_ORIENT_DEGEN, &o[1]);
