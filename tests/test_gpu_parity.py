@@ -161,6 +161,26 @@ def test_near_clique_closures_vs_oracle():
                 assert r.count == exp, (mode, k, gran, r.count, exp)
 
 
+def test_wide_kernel_near_clique_closures_vs_oracle():
+    # node tasks above 256 vertices go to the wide kernel; their h = k-4
+    # entries of 129..256 candidates are compacted to four-word rows and
+    # closed by the complement counts (complement density 0.2: triangles and
+    # 4-cliques in the complement)
+    G = _planted_near(3000, 12000, [(380, 0.8)], seed=9)
+    rp, col = _host_csr(G)
+    pr, _ = O.peel_priority(rp, col)
+    orp, oc = O.orient(rp, col, pr)
+    assert np.diff(orp).max() > 256
+    for k in (5, 6):
+        exp = O.count(orp, oc, k, "pivot_tails", "node")["count"]
+        for mode in ("degree", "degen"):
+            Og = G.orient(mode)
+            for gran in GRANS:
+                r = Og.count(k, "pivot", gran, part_size=1)
+                print(mode, k, gran, r.count, r.raw["count_ms"], r.raw["class_kernel"], flush=True)
+                assert r.count == exp, (mode, k, gran, r.count, exp)
+
+
 def test_measured_autotune_plan_counts():
     # a graph with out-degrees above 64: the plan is measured; counting with
     # it gives the oracle's count
