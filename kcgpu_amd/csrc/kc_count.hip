@@ -20,14 +20,19 @@
 //      orientation: P-lane partitions run independent DFS over the rows, a
 //        child candidate set is the AND of two rows, the last level adds a
 //        popcount (kckernels_orientation.cuh:381-463);
-//      pivoting: one SCT tree per wave; the pivot is the smallest index among
-//        the candidates j maximising |X & row_j|, scored by P-lane partitions;
-//        leaves add C(pivots, k - held) from an exact uint64 table with
-//        overflow bits.  Branches on non-pivot vertices are taken first and
-//        the pivot branch last as a tail call, so the stack never holds more
-//        than k - held + 2 levels (the reference keeps maxDeg levels in HBM).
-// Sizes: tasks are binned by an upper bound of the list size (64, 128, 256,
-// 512, 1024 rows in LDS; larger ones keep the same layout in an HBM slice).
+//      pivoting: one SCT tree per wave (per block of waves above 256
+//        vertices); the pivot is the smallest index among the candidates j
+//        maximising |X & row_j|; leaves add C(pivots, k - held) from an exact
+//        uint64 table with overflow bits.  Branches on non-pivot vertices are
+//        taken first and the pivot branch last as a tail call, so the stack
+//        never holds more than k - held + 2 levels (the reference keeps
+//        maxDeg levels in HBM).  Subtrees are re-encoded into one-, two- or
+//        four-word rows held in registers once their candidate sets shrink,
+//        and the last levels close in form (edges; triangles by lane walks;
+//        3- and 4-cliques of near-cliques from the complement).
+// Sizes: tasks are binned by their exact list size (64, 128, 256 vertices in
+// registers, 512, 1024 in LDS, larger in an HBM slice), each class cut into
+// slices drained largest first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1680,7 +1685,11 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
 // pivots -- plus the chain's last node become entries that the waves take
 // from an LDS counter and finish with their own stacks
 // (pivot_count<false, true>).  The chain is the part of the tree that one
-// wave would otherwise walk alone.
+// wave would otherwise walk alone: the cap is 32 levels when the chain lives
+// in LDS and the whole chain (up to 1024 levels) in the HBM slice.  Blocks
+// have up to 8 waves (512 threads), which leaves the register SCT that
+// pivot_count hands subtrees of <= 256 candidates to its full register
+// budget; each wave has its own LDS areas for those subtrees' rows.
 struct WideLayout {
     uint64_t ctl, list, offs, starts, chain, rows, wave, per_wave, total;
 };
