@@ -7,8 +7,10 @@
 //   kcgpu -m cache -g graph -o degree|degen -r out.kco              (oriented-CSR cache)
 //
 // Every reference flag takes an argument (getopt "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:");
-// -w, -s, -x, -a, -e, -v, -t are accepted for compatibility and do not
-// change the result; -r names the output of a conversion.  A .kco input is
+// -w, -s, -a, -e, -v are accepted for compatibility and do not change the
+// result; -t auto lets the autotuner pick algorithm, granularity and P; -x
+// prints the graph statistics after orientation; -r names the output of a
+// conversion.  A .kco input is
 // an oriented-CSR cache file: it is counted with the orientation it was
 // saved with, without reading, cleaning or orienting again.  Output keeps the reference's lines (Read graph time,
 // Preprocess time, Max Core, the Level/Nodes/Counter line, count time and the
@@ -16,12 +18,15 @@
 // value, plus a `kcliques=` line and one JSON metrics line.
 #include <getopt.h>
 
+#include <algorithm>
 #include <cinttypes>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <ctime>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -66,8 +71,71 @@ void usage()
                  "                      P in 1,2,4,...,64, bitmap (b) or index lists (n); default o8b\n"
                  "    -d <device>       HIP device ordinal (default 0)\n"
                  "    -t auto           let the autotuner pick algorithm, granularity and P\n"
-                 "    -w/-s/-x/-a/-e/-v <arg>  accepted for compatibility\n"
+                 "    -x <arg>          print graph statistics after orientation\n"
+                 "    -w/-s/-a/-e/-v <arg>  accepted for compatibility\n"
                  "    -h <any>          help\n\n");
+}
+
+// -x: statistics of the oriented CSR, printed after orientation as the
+// reference does (src/main.cu:239-246 on the oriented row pointers with
+// nnz = m/2).  Line 1 is MatrixStats (include/utils.cuh:115-184):
+// nnz,rows,cols,min,mean,sd,mode,max,count(min),count(mode),count(max) over
+// the row lengths, where min/mode/max skip empty rows (the reference calls
+// the most frequent length the median and starts max at 1) and mean/sd run
+// over every row.  Then PrintMtarixStruct (:187-222): a 10 x 10 grid of the
+// share of entries per block, in thousandths ("+" below one, "-" for none).
+// Block size n/10 (integer); the reference divides by zero below 10 rows and
+// indexes past the grid when n is not a multiple of 10: here the block size
+// is at least 1 and the last row and column of blocks take the remainder.
+static void print_stats(const kc_oriented *o)
+{
+    kc_oriented_info oi;
+    if (kc_oriented_get_info(o, &oi)) return;
+    const uint32_t n = oi.n;
+    std::vector<uint32_t> rp(n + 1), col(oi.m ? oi.m : 1), key(n ? n : 1);
+    if (kc_oriented_export(o, rp.data(), col.data(), key.data())) return;
+    const uint32_t nnz = (uint32_t)oi.m;
+    std::map<uint32_t, uint32_t> hist;
+    for (uint32_t i = 0; i < n; ++i) ++hist[rp[i + 1] - rp[i]];
+    uint32_t mn = 0, mx = 1, mode = 0, cmin = 0, cmode = 0, cmax = 0;
+    uint64_t total = 0;
+    for (const auto &kv : hist) {
+        if (kv.first == 0) continue;
+        if (mn == 0) {
+            mn = kv.first;
+            cmin = kv.second;
+        }
+        if (cmode < kv.second) {
+            mode = kv.first;
+            cmode = kv.second;
+        }
+        total += (uint64_t)kv.second * kv.first;
+        mx = kv.first;
+        cmax = kv.second;
+    }
+    const double mean = n ? (double)total / n : 0.0;
+    double ss = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double d = (double)(rp[i + 1] - rp[i]) - mean;
+        ss += d * d;
+    }
+    const double sd = n ? std::sqrt(ss / n) : 0.0;
+    std::printf("%u,%u,%u,%u,%.1f,%.1f,%u,%u,%u,%u,%u\n", nnz, n, n, mn, mean, sd, mode, mx, cmin, cmode, cmax);
+    const uint32_t unit = std::max<uint32_t>(1, n / 10);
+    double st[10][10] = {};
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t ri = std::min<uint32_t>(9, i / unit);
+        for (uint32_t e = rp[i]; e < rp[i + 1]; ++e) st[ri][std::min<uint32_t>(9, col[e] / unit)] += 1;
+    }
+    for (int i = 0; i < 10; ++i) {
+        for (int j = 0; j < 10; ++j) {
+            const double perc = nnz ? st[i][j] / nnz : 0.0;
+            if (perc >= 0.001) std::printf("%d,", (int)(1000 * st[i][j] / nnz));
+            else if (perc > 0) std::printf("+,");
+            else std::printf("-,");
+        }
+        std::printf("\n");
+    }
 }
 
 int die(int rc, const char *what)
@@ -83,7 +151,7 @@ int main(int argc, char **argv)
     const char *graph = nullptr, *dst = nullptr;
     std::string task = "kc", orient = "degree", by = "node", spec = "o8b";
     int k = 5, dev = 0;
-    bool autotune = false;
+    bool autotune = false, stats = false;
     int opt;
     while ((opt = getopt(argc, argv, "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:")) >= 0) {
         switch (opt) {
@@ -96,7 +164,8 @@ int main(int argc, char **argv)
         case 'q': spec = optarg; break;
         case 't': autotune = std::strcmp(optarg, "auto") == 0; break;
         case 'r': dst = optarg; break;
-        case 'x': case 'a': case 'v': case 's': case 'e': case 'w': break;
+        case 'x': stats = true; break;
+        case 'a': case 'v': case 's': case 'e': case 'w': break;
         case 'h': usage(); return 0;
         default: usage(); return 1;
         }
@@ -175,6 +244,7 @@ int main(int argc, char **argv)
     kc_oriented_get_info(o, &oi);
     if (omode == KC_ORIENT_DEGEN) std::printf("Max Core = %u\n", oi.max_core);
     stamp("Preprocess time: %f s", now() - t1);
+    if (stats) print_stats(o);
     if (task == "cache") {
         if ((rc = kc_oriented_save(o, dst))) return die(rc, "cache");
         stamp("Wrote oriented CSR to %s", dst);

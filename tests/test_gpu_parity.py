@@ -385,6 +385,32 @@ def test_cli_end_to_end(tmp_path, golden):
             assert f"Max Core = {g['max_core']}" in r.stdout
 
 
+def test_cli_graph_stats(tmp_path):
+    """-x: MatrixStats and the 10 x 10 structure grid of the oriented CSR,
+    checked on a hand-worked graph: edge 5-6 and triangle 7-8-9 on ten
+    vertices, degree order -> rows 5:[6] 7:[8,9] 8:[9]; row lengths
+    {0: 7, 1: 2, 2: 1}, mean 0.4, sd sqrt(4.4/10) = 0.66; every entry is a
+    quarter of the four (250 thousandths)."""
+    import os
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kcgpu_amd", "bin", "kcgpu")
+    p = tmp_path / "g.tsv"
+    p.write_text("5 6\n7 8\n7 9\n8 9\n")
+    r = subprocess.run([cli, "-g", str(p), "-m", "kc", "-o", "degree", "-k", "3", "-p", "node", "-q", "p1b",
+                        "-x", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    i = out.index("4,10,10,1,0.4,0.7,1,2,2,2,1")
+    grid = out[i + 1:i + 11]
+    empty = ",".join("-" * 10) + ","
+    want = [empty] * 10
+    want[5] = "-,-,-,-,-,-,250,-,-,-,"
+    want[7] = "-,-,-,-,-,-,-,-,250,250,"
+    want[8] = "-,-,-,-,-,-,-,-,-,250,"
+    assert grid == want, "\n".join(grid)
+    assert "kcliques=1 k=3 overflow=0" in r.stdout
+
+
 def test_oriented_cache_round_trip(tmp_path, golden):
     """save -> load gives the same oriented CSR, key and counts; the CLI counts
     a .kco file without orienting again."""
