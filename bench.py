@@ -474,7 +474,8 @@ def main():
         log(f"warmup {i}")
         step()
     times, dev_ms, last = [], {k: [] for k in ks}, None
-    dom_cls = {k: {"ms": [[] for _ in range(6)], "bytes": [[] for _ in range(6)]} for k in ks}
+    dom_cls = {k: {"ms": [[] for _ in range(6)], "bytes": [[] for _ in range(6)], "wave_ms": [[] for _ in range(6)]}
+               for k in ks}
     t_all = time.perf_counter()
     for i in range(args.steps):
         t = time.perf_counter()
@@ -486,6 +487,7 @@ def main():
             for c in range(6):
                 dom_cls[k]["ms"][c].append(last[k].raw["class_ms"][c])
                 dom_cls[k]["bytes"][c].append(last[k].raw["class_bytes"][c])
+                dom_cls[k]["wave_ms"][c].append(last[k].raw["class_task_ms_sum"][c])
         log(f"step {i}: {times[-1]:.3f}s", {k: last[k].count for k in ks})
     wall = time.perf_counter() - t_all
     per_step = wall / args.steps
@@ -499,10 +501,13 @@ def main():
                          "edges_per_s": m_or / (ms / 1e3) if ms else None, "tasks": last[k].raw["tasks"],
                          "tree_nodes": last[k].raw["tree_nodes"], "part_size": last[k].raw["part_size"]}
 
-    # dominant kernel symbol: the size class with the most device time over the
-    # step (one kernel symbol per class), averaged over its launches
+    # dominant kernel symbol: the size class whose tasks hold the device's
+    # waves longest over the step (summed per-task wave time; the launches of
+    # the small classes overlap, so their durations all span most of a
+    # count), with its launch duration averaged over its launches
     cls_ms = np.zeros(6)
     cls_bytes = np.zeros(6)
+    cls_wave = np.zeros(6)
     cls_launch = np.zeros(6)
     cls_var = [0] * 6
     for k in ks:
@@ -510,9 +515,10 @@ def main():
             if last[k].raw["class_kernel"][c]:
                 cls_ms[c] += np.mean([x for x in dom_cls[k]["ms"][c]])
                 cls_bytes[c] += np.mean([x for x in dom_cls[k]["bytes"][c]])
+                cls_wave[c] += np.mean([x for x in dom_cls[k]["wave_ms"][c]])
                 cls_launch[c] += 1
                 cls_var[c] = last[k].raw["class_kernel"][c]
-    dc = int(np.argmax(cls_ms))
+    dc = int(np.argmax(cls_wave))
     avg_ms = cls_ms[dc] / max(cls_launch[dc], 1)
     avg_bytes = cls_bytes[dc] / max(cls_launch[dc], 1)
     achieved = avg_bytes / (avg_ms / 1e3) / 1e9 if avg_ms else 0.0

@@ -2172,8 +2172,34 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     }
 
     // --- one persistent launch per non-empty bin, largest tasks first ---
-    hipEvent_t ev[NB];
+    // The bins of tasks of at most 256 vertices run side by side on the
+    // device's side streams (each bin has its own queue head and
+    // accumulators), so the waves a bin frees while its last long tasks
+    // finish take the next bin's tasks.  Larger classes (the wide kernel,
+    // launches that use the shared HBM slice gscr) run first and in order on
+    // s, and the side streams start after them: sharing the device with the
+    // small classes delays the wide classes' long chain walks.  With
+    // grid_share > 1 (a server worker) every launch stays on s.
+    hipEvent_t ev[NB], ev0[NB], e_go, e_fan, e_end;
     for (auto &e : ev) hipEventCreate(&e);
+    for (auto &e : ev0) hipEventCreate(&e);
+    hipEventCreate(&e_go);
+    hipEventCreate(&e_fan);
+    hipEventCreate(&e_end);
+    auto destroy_events = [&]() {
+        for (auto &e : ev) hipEventDestroy(e);
+        for (auto &e : ev0) hipEventDestroy(e);
+        hipEventDestroy(e_go);
+        hipEventDestroy(e_fan);
+        hipEventDestroy(e_end);
+        hipEventDestroy(e_start);
+        hipEventDestroy(e_cls);
+    };
+    hipEventRecord(e_go, s);
+    const bool fan = grid_share == 1;
+    bool side_waits[DeviceCtx::NSIDE] = {false}, fan_recorded = false;
+    hipStream_t bin_stream[NB];
+    int nside = 0;
     uint32_t variant[NB] = {0};
     bool launched[NB] = {false};
     const size_t lds_cap = ctx->lds_per_block;
@@ -2182,7 +2208,6 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         order[no++] = NCLASS + c;  // pivot bins
         order[no++] = c;           // orientation bins
     }
-    hipEvent_t prev = e_cls;
     for (int oi = 0; oi < NB; ++oi) {
         const int b = order[oi];
         if (bcnt[b] == 0) continue;
@@ -2232,9 +2257,9 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             // whole chain for the HBM classes (a chain left whole is one entry,
             // which one wave would walk alone), 32 levels when LDS holds them
             if (a.rmax >= 0xFFFFu) {
-                for (auto &e : ev) hipEventDestroy(e);
-                hipEventDestroy(e_start);
-                hipEventDestroy(e_cls);
+                hipStreamSynchronize(s);
+                for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
+                destroy_events();
                 return fail(KC_ERR_LIMIT, "a task of %u vertices exceeds the wide kernel's 65534", a.rmax);
             }
             a.chain = c == 3 ? 32u : std::min<uint32_t>(1024u, std::max<uint32_t>(64u, a.rmax));
@@ -2270,16 +2295,36 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             const uint64_t cap = (2ull << 30) / (L.total * 8 + 1);
             blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
             if ((rc = scr.gscr.ensure(blocks * L.total * 8))) {
-                for (auto &e : ev) hipEventDestroy(e);
+                hipStreamSynchronize(s);
+                for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
+                destroy_events();
                 return rc;
             }
             a.gscr = scr.gscr.as<uint64_t>();
         }
+        hipStream_t st = s;
+        if (fan && !global && c <= 2) {
+            if (!fan_recorded) {
+                KC_HIP(hipEventRecord(e_fan, s));  // after every launch so far on s
+                fan_recorded = true;
+            }
+            const int si = nside++ % DeviceCtx::NSIDE;
+            st = ctx->side[si];
+            if (!side_waits[si]) {
+                KC_HIP(hipStreamWaitEvent(st, e_fan, 0));
+                side_waits[si] = true;
+            }
+        }
+        bin_stream[b] = st;
         void *args[] = {&a};
-        KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, s));
-        hipEventRecord(ev[b], s);
+        hipEventRecord(ev0[b], st);
+        KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, st));
+        hipEventRecord(ev[b], st);
         launched[b] = true;
     }
+    for (int b = 0; b < NB; ++b)
+        if (launched[b] && bin_stream[b] != s) KC_HIP(hipStreamWaitEvent(s, ev[b], 0));
+    hipEventRecord(e_end, s);
     std::vector<uint64_t> hq_count(nq);
     std::vector<uint32_t> hq_ovf(nq);
     uint64_t hacc[NB * ACC];
@@ -2288,23 +2333,15 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     KC_HIP(hipMemcpyAsync(hq_ovf.data(), qovf, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, s));
     const hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) {
-        for (auto &e : ev) hipEventDestroy(e);
-        hipEventDestroy(e_start);
-        hipEventDestroy(e_cls);
+        destroy_events();
         return fail(KC_ERR_DEVICE, "counting kernels failed: %s", hipGetErrorString(se));
     }
     float bin_ms[NB] = {0}, total_ms = 0, prep_ms = 0;
     hipEventElapsedTime(&prep_ms, e_start, e_cls);
-    for (int oi = 0; oi < NB; ++oi) {
-        const int b = order[oi];
-        if (!launched[b]) continue;
-        hipEventElapsedTime(&bin_ms[b], prev, ev[b]);
-        total_ms += bin_ms[b];
-        prev = ev[b];
-    }
-    for (auto &e : ev) hipEventDestroy(e);
-    hipEventDestroy(e_start);
-    hipEventDestroy(e_cls);
+    hipEventElapsedTime(&total_ms, e_go, e_end);
+    for (int b = 0; b < NB; ++b)
+        if (launched[b]) hipEventElapsedTime(&bin_ms[b], ev0[b], ev[b]);
+    destroy_events();
     int wall_khz = 100000;
     hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ctx->dev);
     if (wall_khz <= 0) wall_khz = 100000;

@@ -27,6 +27,11 @@ struct DeviceCtx {
     size_t hbm = 0;
     hipDeviceProp_t prop{};
     hipStream_t stream = nullptr;  // library stream (default priority), shared by handles
+    // streams (default priority) on which the size-class launches of one count
+    // run side by side, so that one class's last long tasks overlap the next
+    // class instead of leaving the device idle; joined back into `stream`
+    static constexpr int NSIDE = 3;
+    hipStream_t side[NSIDE] = {nullptr, nullptr, nullptr};
 };
 
 // Select and check the device (gfx950).  Returns KC_OK or KC_ERR_DEVICE.

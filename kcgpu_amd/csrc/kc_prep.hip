@@ -69,7 +69,9 @@ int device_ctx(DeviceCtx **out)
                 ctx.lds_per_block = p.sharedMemPerBlock;
                 ctx.hbm = p.totalGlobalMem;
                 hipSetDevice(d);
-                if (!ctx.stream && hipStreamCreate(&ctx.stream) != hipSuccess) {
+                bool ok = ctx.stream || hipStreamCreate(&ctx.stream) == hipSuccess;
+                for (hipStream_t &st : ctx.side) ok = ok && (st || hipStreamCreate(&st) == hipSuccess);
+                if (!ok) {
                     state = 2;
                     why = "hipStreamCreate failed";
                 } else {
@@ -756,7 +758,10 @@ int kc_select_device(int ordinal)
     {
         std::lock_guard<std::mutex> lk(g_mu);
         if (hipSetDevice(ordinal) != hipSuccess) return fail(KC_ERR_DEVICE, "hipSetDevice(%d) failed", ordinal);
-        if (g_state == 1 && g_ctx.dev != ordinal) g_ctx.stream = nullptr;  // the old device keeps its stream
+        if (g_state == 1 && g_ctx.dev != ordinal) {  // the old device keeps its streams
+            g_ctx.stream = nullptr;
+            for (hipStream_t &st : g_ctx.side) st = nullptr;
+        }
         g_state = 0;  // re-read the properties of the selected device
     }
     DeviceCtx *ctx;
