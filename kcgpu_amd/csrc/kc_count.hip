@@ -2195,6 +2195,21 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         hipEventDestroy(e_start);
         hipEventDestroy(e_cls);
     };
+    // an error after the first launch waits for every launch before it
+    // returns, so that no kernel still runs on the caller's scratch
+    auto drain = [&]() {
+        hipStreamSynchronize(s);
+        for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
+        destroy_events();
+    };
+#define KC_HIP_L(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            drain();                                                                               \
+            return kc::fail(KC_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));         \
+        }                                                                                          \
+    } while (0)
     hipEventRecord(e_go, s);
     const bool fan = grid_share == 1;
     bool side_waits[DeviceCtx::NSIDE] = {false}, fan_recorded = false;
@@ -2257,9 +2272,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             // whole chain for the HBM classes (a chain left whole is one entry,
             // which one wave would walk alone), 32 levels when LDS holds them
             if (a.rmax >= 0xFFFFu) {
-                hipStreamSynchronize(s);
-                for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
-                destroy_events();
+                drain();
                 return fail(KC_ERR_LIMIT, "a task of %u vertices exceeds the wide kernel's 65534", a.rmax);
             }
             a.chain = c == 3 ? 32u : std::min<uint32_t>(1024u, std::max<uint32_t>(64u, a.rmax));
@@ -2283,9 +2296,9 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             variant[b] = global ? 5u : 4u;
         }
         const size_t lds = global ? 0 : L.total * 8;
-        if (!global) KC_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (!global) KC_HIP_L(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int per_cu = 0;
-        KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(64 * nw), lds));
+        KC_HIP_L(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(64 * nw), lds));
         if (per_cu < 1) per_cu = 1;
         uint64_t blocks = std::max<uint64_t>(1, (uint64_t)per_cu * ctx->cu / std::max<uint32_t>(1, grid_share));
         a.chunk = (c == 0 && bcnt[b] > blocks * 64) ? 4 : 1;  // (the wide kernel takes one task per grab)
@@ -2295,9 +2308,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             const uint64_t cap = (2ull << 30) / (L.total * 8 + 1);
             blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
             if ((rc = scr.gscr.ensure(blocks * L.total * 8))) {
-                hipStreamSynchronize(s);
-                for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
-                destroy_events();
+                drain();
                 return rc;
             }
             a.gscr = scr.gscr.as<uint64_t>();
@@ -2305,37 +2316,38 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         hipStream_t st = s;
         if (fan && !global && c <= 2) {
             if (!fan_recorded) {
-                KC_HIP(hipEventRecord(e_fan, s));  // after every launch so far on s
+                KC_HIP_L(hipEventRecord(e_fan, s));  // after every launch so far on s
                 fan_recorded = true;
             }
             const int si = nside++ % DeviceCtx::NSIDE;
             st = ctx->side[si];
             if (!side_waits[si]) {
-                KC_HIP(hipStreamWaitEvent(st, e_fan, 0));
+                KC_HIP_L(hipStreamWaitEvent(st, e_fan, 0));
                 side_waits[si] = true;
             }
         }
         bin_stream[b] = st;
         void *args[] = {&a};
         hipEventRecord(ev0[b], st);
-        KC_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, st));
+        KC_HIP_L(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, st));
         hipEventRecord(ev[b], st);
         launched[b] = true;
     }
     for (int b = 0; b < NB; ++b)
-        if (launched[b] && bin_stream[b] != s) KC_HIP(hipStreamWaitEvent(s, ev[b], 0));
+        if (launched[b] && bin_stream[b] != s) KC_HIP_L(hipStreamWaitEvent(s, ev[b], 0));
     hipEventRecord(e_end, s);
     std::vector<uint64_t> hq_count(nq);
     std::vector<uint32_t> hq_ovf(nq);
     uint64_t hacc[NB * ACC];
-    KC_HIP(hipMemcpyAsync(hacc, dacc, sizeof hacc, hipMemcpyDeviceToHost, s));
-    KC_HIP(hipMemcpyAsync(hq_count.data(), qcount, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, s));
-    KC_HIP(hipMemcpyAsync(hq_ovf.data(), qovf, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, s));
+    KC_HIP_L(hipMemcpyAsync(hacc, dacc, sizeof hacc, hipMemcpyDeviceToHost, s));
+    KC_HIP_L(hipMemcpyAsync(hq_count.data(), qcount, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, s));
+    KC_HIP_L(hipMemcpyAsync(hq_ovf.data(), qovf, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, s));
     const hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) {
         destroy_events();
         return fail(KC_ERR_DEVICE, "counting kernels failed: %s", hipGetErrorString(se));
     }
+#undef KC_HIP_L
     float bin_ms[NB] = {0}, total_ms = 0, prep_ms = 0;
     hipEventElapsedTime(&prep_ms, e_start, e_cls);
     hipEventElapsedTime(&total_ms, e_go, e_end);

@@ -127,6 +127,24 @@ def test_size_classes_vs_oracle(mode):
                 assert r.count == exp, (mode, k, algo, gran)
 
 
+def test_register_classes_side_by_side_vs_oracle():
+    # tasks in all three register classes (<= 64, <= 128, <= 256 vertices), whose
+    # launches run concurrently on separate streams; repeated counts must agree
+    # with the oracle every time
+    G = _planted(4000, 16000, [100, 100, 200, 200], seed=11)
+    rp, col = _host_csr(G)
+    orp, oc = O.orient(rp, col, O.peel_priority(rp, col)[0])
+    Og = G.orient("degen")
+    for k in (5, 6):
+        for gran in GRANS:
+            exp = O.count(orp, oc, k, "pivot", gran)["count"]
+            for rep in range(3):
+                r = Og.count(k, "pivot", gran, part_size=1)
+                print(k, gran, rep, r.count, r.raw["class_tasks"][:3], flush=True)
+                assert all(t > 0 for t in r.raw["class_tasks"][:3]), r.raw["class_tasks"]
+                assert r.count == exp, (k, gran, rep)
+
+
 def _planted_near(n, m, specs, seed):
     """sparse ER with planted near-cliques (size, edge density): dense candidate
     sets of 65..256 vertices whose complements hold triangles and 4-cliques"""
