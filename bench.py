@@ -10,8 +10,9 @@ aggregated over the step: 4 * m_oriented / (wall time of the four counts).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ks 5,6,7,8] [--no-cpu]
 
 Prints one JSON line.  The roofline object describes the dominant kernel (the
-k=8 launch of the largest-time size class), timed with HIP events on the
-library's stream; the cpu_baseline leg times the CPU port of the same
+size class whose tasks hold the device's waves longest over the step),
+averaged over its launches, each timed with HIP events on the stream it runs
+on; the cpu_baseline leg times the CPU port of the same
 algorithm (oracle/, closed-form SCT tails) on a strided sample of edge tasks.
 """
 from __future__ import annotations
