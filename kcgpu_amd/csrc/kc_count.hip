@@ -2172,7 +2172,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     }
 
     // --- one persistent launch per non-empty bin, largest tasks first ---
-    // The bins of tasks of at most 256 vertices run side by side on the
+    // The bins of tasks of at most 256 vertices can run side by side on the
     // device's side streams (each bin has its own queue head and
     // accumulators), so the waves a bin frees while its last long tasks
     // finish take the next bin's tasks.  Larger classes (the wide kernel,
@@ -2211,7 +2211,16 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         }                                                                                          \
     } while (0)
     hipEventRecord(e_go, s);
-    const bool fan = grid_share == 1;
+    // Side by side only when the register classes hold few tasks per CU: then
+    // a class's last long tasks are a large part of its launch (C2 at k=8:
+    // 8,600 tasks per CU, one 125 ms task in a 132 ms launch; 572 -> 526 ms).
+    // With many tasks per CU the tail is negligible and overlapping launches
+    // only contend (C3 edge tasks at k=5: 166,000 per CU, 3.69 -> 3.91 s).
+    constexpr uint64_t FAN_MAX_TASKS_PER_CU = 1u << 15;
+    uint64_t reg_tasks = 0;
+    for (int b = 0; b < NB; ++b)
+        if (b % NCLASS <= 2) reg_tasks += bcnt[b];
+    const bool fan = grid_share == 1 && reg_tasks < FAN_MAX_TASKS_PER_CU * (uint64_t)ctx->cu;
     bool side_waits[DeviceCtx::NSIDE] = {false}, fan_recorded = false;
     hipStream_t bin_stream[NB];
     int nside = 0;
