@@ -2212,10 +2212,11 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     } while (0)
     hipEventRecord(e_go, s);
     // Side by side only when the register classes hold few tasks per CU: then
-    // a class's last long tasks are a large part of its launch (C2 at k=8:
-    // 8,600 tasks per CU, one 125 ms task in a 132 ms launch; 572 -> 526 ms).
-    // With many tasks per CU the tail is negligible and overlapping launches
-    // only contend (C3 edge tasks at k=5: 166,000 per CU, 3.69 -> 3.91 s).
+    // a class's last long tasks are a large part of its launch (Chung-Lu
+    // n=1.7M m=11.1M, edge tasks at k=8: 8,600 per CU, one 125 ms task in a
+    // 132 ms launch; the count 572 -> 526 ms).  With many tasks per CU the tail
+    // is negligible and overlapping launches only contend (R-MAT scale 22,
+    // edge tasks at k=5: 166,000 per CU, 3.69 -> 3.91 s).
     constexpr uint64_t FAN_MAX_TASKS_PER_CU = 1u << 15;
     uint64_t reg_tasks = 0;
     for (int b = 0; b < NB; ++b)
