@@ -405,6 +405,59 @@ def run_copy(args):
                       "GBps_read_plus_write": 3 * 2 * nbytes / dt / 1e9}), flush=True)
 
 
+def host_cores():
+    """(threads the CPU leg uses, logical CPUs this process may run on,
+    physical cores of the host from lscpu or None).  The CPU leg keeps to the
+    16 host threads a one-GPU share of the machine gives."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    phys = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        phys = len({l for l in out.splitlines() if l and not l.startswith("#")}) or None
+    except Exception:
+        pass
+    return min(16, usable), usable, phys
+
+
+def cpu_baseline(K, Og, ks, m_or, stride, repeats):
+    """The oracle's enumerator (pivoting with the same closed-form tails, C
+    with OpenMP) over every stride-th oriented edge task of the same oriented
+    CSR, timed as the median of `repeats` runs; the GPU count over the same
+    tasks (kc_count_sampled) must equal the oracle's at every k."""
+    import numpy as np
+    from oracle import oracle as O
+    orp, oc, _ = Og.export()
+    nt, usable, phys = host_cores()
+    runs, counts = [], {}
+    for rep in range(repeats):
+        t = time.perf_counter()
+        for k in ks:
+            r = O.count(orp, oc, k, "pivot_tails", "edge", stride=stride, offset=0, threads=nt)
+            counts[k] = r
+        runs.append(time.perf_counter() - t)
+        log(f"cpu run {rep}: {runs[-1]:.2f}s")
+    tc = float(np.median(runs))
+    parity = {}
+    for k in ks:
+        g = Og.count_sampled(k, stride, 0, "pivot", "edge")
+        parity[str(k)] = {"gpu": g.count, "oracle": counts[k]["count"], "tasks": counts[k]["tasks"]}
+        if g.count != counts[k]["count"] or g.raw["tasks"] != counts[k]["tasks"] or g.overflow:
+            raise SystemExit(f"sampled count differs from the oracle at k={k}: gpu {g.count} "
+                             f"({g.raw['tasks']} tasks) oracle {counts[k]['count']} ({counts[k]['tasks']} tasks)")
+    tasks = sum(counts[k]["tasks"] for k in ks)
+    sample_edges = (m_or + stride - 1) // stride
+    return {"value": len(ks) * sample_edges / tc, "unit": "edges/s", "cores": nt, "kind": "port",
+            "sample": f"every {stride}th oriented edge ({sample_edges} edges, {tasks} tasks with |T|>=k-2) "
+                      f"at k={','.join(map(str, ks))}; median of {repeats} runs on {nt} OpenMP threads",
+            "seconds": round(tc, 2), "runs_s": [round(x, 2) for x in runs],
+            "host": {"usable_logical_cpus": usable, "physical_cores": phys},
+            "parity": {"stride": stride, "offset": 0, "per_k": parity, "equal": True}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,7 +466,8 @@ def main():
     ap.add_argument("--ks", default="5,6,7,8")
     ap.add_argument("--part-size", type=int, default=0, help="0 = autotune")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=30)
+    ap.add_argument("--cpu-stride", type=int, default=60)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--workload", choices=["c2", "batch", "serve", "copy", "c1", "c3", "lkc", "formats"], default="c2",
                     help="c2: the headline k=5..8 count (default); batch: the 4096-graph batched mix; "
@@ -539,21 +593,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        orp, oc, _ = Og.export()
-        nt = min(16, os.cpu_count() or 1)
-        stride = args.cpu_stride
-        t = time.perf_counter()
-        tasks = 0
-        for k in ks:
-            r = O.count(orp, oc, k, "pivot_tails", "edge", stride=stride, offset=0, threads=nt)
-            tasks += r["tasks"]
-        tc = time.perf_counter() - t
-        sample_edges = (m_or + stride - 1) // stride
-        cpu = {"value": len(ks) * sample_edges / tc, "unit": "edges/s", "cores": nt, "kind": "port",
-               "sample": f"every {stride}th oriented edge ({sample_edges} edges, {tasks} tasks with |T|>=k-2) "
-                         f"at k={','.join(map(str, ks))}; {tc:.1f}s on {nt} OpenMP threads",
-               "seconds": round(tc, 2)}
+        cpu = cpu_baseline(K, Og, ks, m_or, args.cpu_stride, args.cpu_repeats)
         log("cpu baseline", cpu)
 
     out = {"metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,

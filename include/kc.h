@@ -235,6 +235,19 @@ typedef struct {
 int kc_count(const kc_oriented *o, const kc_query *q, kc_result *result);
 
 /*
+ * kc_count over a task sample: only the tasks whose id is congruent to
+ * offset modulo stride run (node tasks are numbered by vertex id, edge tasks
+ * by oriented edge in CSR order; 0 <= offset < stride, stride 1 = kc_count).
+ * The tasks a graph has are the reference's task list (node tasks:
+ * kclique/kclique.cuh:1318-1361, edge tasks: :1365-1406); the sample lets a
+ * CPU enumerator count the same tasks of a graph too large to count whole on
+ * the host, so the two counts must be equal.  The result's count, tasks and
+ * alg_bytes cover the sampled tasks only.
+ */
+int kc_count_sampled(const kc_oriented *o, const kc_query *q, uint32_t stride, uint32_t offset,
+                     kc_result *result);
+
+/*
  * Per-vertex counts (-m lkc): local[v] = number of k-cliques that contain v,
  * for every v < n (local has n entries).  result->count is the total count
  * (the sum of local[] is k times it); no entry can wrap unless the total

@@ -133,6 +133,8 @@ SIGNATURES = {
     "kc_oriented_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "kc_convert": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint]),
     "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
+    "kc_count_sampled": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(Result)]),
     "kc_count_local": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(Result)]),
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
@@ -342,6 +344,17 @@ class Oriented:
                   int(bool(auto_tune)))
         r = Result()
         rc = lib().kc_count(self._h, ctypes.byref(q), ctypes.byref(r))
+        if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
+            _check(rc)
+        return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+    def count_sampled(self, k, stride, offset=0, algo="pivot", gran="node", part_size=1, binary=True,
+                      auto_tune=False, check=True):
+        """kc_count over the tasks whose id % stride == offset (vertex ids, or
+        oriented edges in CSR order): the sample a CPU enumerator can also count."""
+        q = _query(k, algo, gran, part_size, binary, auto_tune)
+        r = Result()
+        rc = lib().kc_count_sampled(self._h, ctypes.byref(q), int(stride), int(offset), ctypes.byref(r))
         if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
             _check(rc)
         return CountResult(int(r.count), bool(r.overflow), r.as_dict())

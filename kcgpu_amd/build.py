@@ -22,8 +22,8 @@ BIN = os.path.join(HERE, "bin", "kcgpu")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-CPP = ["kc_io.cpp", "kc_gen.cpp", "kc_api.cpp", "kc_convert.cpp"]
-HIP = ["kc_prep.hip", "kc_count.hip", "kc_serve.hip"]
+CPP = sorted(f for f in os.listdir(CSRC) if f.endswith(".cpp"))
+HIP = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -51,7 +51,7 @@ constexpr int ACC = 8;  // per-class accumulator slots
 // one query of a launch: the oriented graph it runs on and its k
 struct QDesc {
     const uint32_t *rowptr, *col, *src;
-    uint32_t k, gran, sample, pad;
+    uint32_t k, gran, sample, offset;  // tasks with id % sample == offset run
 };
 
 // the graph and k of the task a wave is working on
@@ -1987,7 +1987,7 @@ __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t
             const QDesc d = qd[q];
             const uint32_t need = d.gran ? d.k - 2 : d.k - 1;
             s = size[gi];
-            if (s >= need && (d.sample <= 1 || (uint32_t)(gi - qoff[q]) % d.sample == 0)) {
+            if (s >= need && (d.sample <= 1 || (uint32_t)(gi - qoff[q]) % d.sample == d.offset)) {
                 const int c = class_of(s);
                 fb = ((qalgo[q] ? NCLASS : 0) + c) * NSUB + slice_of(s, c);
             }
@@ -2061,6 +2061,54 @@ static void binom_table(uint32_t nmax, uint32_t rmax, std::vector<uint64_t> &val
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// The HIP events of one run_queries call, owned here so that every return
+// path destroys them.  With a fixed scratch (no allocation of any kind) there
+// are none: records and elapsed times become no-ops and the launches stay on
+// one stream (the side-by-side launches need events to join).
+struct RunEvents {
+    static constexpr int NB = 2 * NCLASS;
+    hipEvent_t start = nullptr, cls = nullptr, go = nullptr, fan = nullptr, end = nullptr;
+    hipEvent_t b0[NB] = {}, b1[NB] = {};
+    bool on = false;
+    int create()
+    {
+        on = true;
+        hipEvent_t *all[5] = {&start, &cls, &go, &fan, &end};
+        for (hipEvent_t *e : all) KC_HIP(hipEventCreate(e));
+        for (int b = 0; b < NB; ++b) {
+            KC_HIP(hipEventCreate(&b0[b]));
+            KC_HIP(hipEventCreate(&b1[b]));
+        }
+        return KC_OK;
+    }
+    void record(hipEvent_t e, hipStream_t st) const
+    {
+        if (on) hipEventRecord(e, st);
+    }
+    float ms(hipEvent_t a, hipEvent_t b) const
+    {
+        float t = 0;
+        if (on) hipEventElapsedTime(&t, a, b);
+        return t;
+    }
+    ~RunEvents()
+    {
+        hipEvent_t all[5] = {start, cls, go, fan, end};
+        for (hipEvent_t e : all)
+            if (e) hipEventDestroy(e);
+        for (int b = 0; b < NB; ++b) {
+            if (b0[b]) hipEventDestroy(b0[b]);
+            if (b1[b]) hipEventDestroy(b1[b]);
+        }
+    }
+};
+
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
                 LaunchStats *ls, uint64_t *local, uint32_t grid_share)
 {
@@ -2082,7 +2130,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         const kc_oriented *o = qs[i].o;
         const kc_query &q = qs[i].q;
         hq[i] = QDesc{o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(), o->src.as<uint32_t>(), q.k, q.gran,
-                      std::max<uint32_t>(1, qs[i].sample), 0};
+                      std::max<uint32_t>(1, qs[i].sample), qs[i].sample > 1 ? qs[i].offset : 0};
         halgo[i] = (uint8_t)(q.algo == KC_ALGO_PIVOT);
         hoff[i + 1] = hoff[i] + (q.gran == KC_GRAN_EDGE ? o->m : o->n);
         kmax = std::max(kmax, q.k);
@@ -2111,12 +2159,15 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     KC_HIP(hipMemcpyAsync(dq, hq.data(), qd_bytes, hipMemcpyHostToDevice, s));
     KC_HIP(hipMemcpyAsync(doff, hoff.data(), off_bytes, hipMemcpyHostToDevice, s));
     KC_HIP(hipMemcpyAsync(dalgo, halgo.data(), nq, hipMemcpyHostToDevice, s));
-    KC_HIP(hipMemsetAsync(scr.meta.p, 0, meta_u32 * 4 + acc_bytes + q_bytes, s));
+    {  // counters, queue heads and per-query sums start at zero (a fill kernel, no memset node)
+        const uint64_t nz = (meta_u32 * 4 + acc_bytes + q_bytes) / 4;
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<uint64_t>(1024, (nz + 255) / 256)), dim3(256), 0, s,
+                           m32, nz, 0u);
+    }
 
-    hipEvent_t e_start, e_cls;
-    hipEventCreate(&e_start);
-    hipEventCreate(&e_cls);
-    hipEventRecord(e_start, s);
+    RunEvents evs;
+    if (!scr.fixed && (rc = evs.create())) return rc;
+    evs.record(evs.start, s);
     const int gs = (int)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, (uint64_t)ctx->cu * 16));
     // task sizes of a single query on the graph they were last computed for are reused
     const bool sizes_cached = nq == 1 && scr.size_uid == qs[0].o->uid && scr.size_gran == qs[0].q.gran &&
@@ -2156,7 +2207,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
                            sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 1);
     KC_HIP(hipGetLastError());
-    hipEventRecord(e_cls, s);
+    evs.record(evs.cls, s);
 
     // --- binomial table for the pivot leaves ---
     const uint32_t bstride = kmax + 1;
@@ -2180,27 +2231,11 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     // s, and the side streams start after them: sharing the device with the
     // small classes delays the wide classes' long chain walks.  With
     // grid_share > 1 (a server worker) every launch stays on s.
-    hipEvent_t ev[NB], ev0[NB], e_go, e_fan, e_end;
-    for (auto &e : ev) hipEventCreate(&e);
-    for (auto &e : ev0) hipEventCreate(&e);
-    hipEventCreate(&e_go);
-    hipEventCreate(&e_fan);
-    hipEventCreate(&e_end);
-    auto destroy_events = [&]() {
-        for (auto &e : ev) hipEventDestroy(e);
-        for (auto &e : ev0) hipEventDestroy(e);
-        hipEventDestroy(e_go);
-        hipEventDestroy(e_fan);
-        hipEventDestroy(e_end);
-        hipEventDestroy(e_start);
-        hipEventDestroy(e_cls);
-    };
     // an error after the first launch waits for every launch before it
     // returns, so that no kernel still runs on the caller's scratch
     auto drain = [&]() {
         hipStreamSynchronize(s);
         for (hipStream_t st : ctx->side) hipStreamSynchronize(st);
-        destroy_events();
     };
 #define KC_HIP_L(call)                                                                              \
     do {                                                                                           \
@@ -2210,7 +2245,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             return kc::fail(KC_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));         \
         }                                                                                          \
     } while (0)
-    hipEventRecord(e_go, s);
+    evs.record(evs.go, s);
     // Side by side only when the register classes hold few tasks per CU: then
     // a class's last long tasks are a large part of its launch (Chung-Lu
     // n=1.7M m=11.1M, edge tasks at k=8: 8,600 per CU, one 125 ms task in a
@@ -2221,7 +2256,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     uint64_t reg_tasks = 0;
     for (int b = 0; b < NB; ++b)
         if (b % NCLASS <= 2) reg_tasks += bcnt[b];
-    const bool fan = grid_share == 1 && reg_tasks < FAN_MAX_TASKS_PER_CU * (uint64_t)ctx->cu;
+    const bool fan = evs.on && grid_share == 1 && reg_tasks < FAN_MAX_TASKS_PER_CU * (uint64_t)ctx->cu;
     bool side_waits[DeviceCtx::NSIDE] = {false}, fan_recorded = false;
     hipStream_t bin_stream[NB];
     int nside = 0;
@@ -2315,7 +2350,13 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         blocks = std::min<uint64_t>(blocks, ((uint64_t)bcnt[b] + a.chunk - 1) / a.chunk);
         if (global) {
             a.gscr_words = L.total;
-            const uint64_t cap = (2ull << 30) / (L.total * 8 + 1);
+            // a fixed scratch holds as many slices as fit in what it has left
+            const uint64_t cap = (scr.fixed ? scr.gscr.bytes : (2ull << 30)) / (L.total * 8 + 1);
+            if (scr.fixed && cap == 0) {
+                drain();
+                return fail(KC_ERR_LIMIT, "a task of %u vertices needs %llu bytes of HBM workspace, more than the "
+                            "%zu left in the fixed scratch", a.rmax, (unsigned long long)(L.total * 8), scr.gscr.bytes);
+            }
             blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
             if ((rc = scr.gscr.ensure(blocks * L.total * 8))) {
                 drain();
@@ -2326,26 +2367,26 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         hipStream_t st = s;
         if (fan && !global && c <= 2) {
             if (!fan_recorded) {
-                KC_HIP_L(hipEventRecord(e_fan, s));  // after every launch so far on s
+                KC_HIP_L(hipEventRecord(evs.fan, s));  // after every launch so far on s
                 fan_recorded = true;
             }
             const int si = nside++ % DeviceCtx::NSIDE;
             st = ctx->side[si];
             if (!side_waits[si]) {
-                KC_HIP_L(hipStreamWaitEvent(st, e_fan, 0));
+                KC_HIP_L(hipStreamWaitEvent(st, evs.fan, 0));
                 side_waits[si] = true;
             }
         }
         bin_stream[b] = st;
         void *args[] = {&a};
-        hipEventRecord(ev0[b], st);
+        evs.record(evs.b0[b], st);
         KC_HIP_L(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(64 * nw), args, lds, st));
-        hipEventRecord(ev[b], st);
+        evs.record(evs.b1[b], st);
         launched[b] = true;
     }
     for (int b = 0; b < NB; ++b)
-        if (launched[b] && bin_stream[b] != s) KC_HIP_L(hipStreamWaitEvent(s, ev[b], 0));
-    hipEventRecord(e_end, s);
+        if (launched[b] && bin_stream[b] != s) KC_HIP_L(hipStreamWaitEvent(s, evs.b1[b], 0));
+    evs.record(evs.end, s);
     std::vector<uint64_t> hq_count(nq);
     std::vector<uint32_t> hq_ovf(nq);
     uint64_t hacc[NB * ACC];
@@ -2353,17 +2394,12 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     KC_HIP_L(hipMemcpyAsync(hq_count.data(), qcount, sizeof(uint64_t) * nq, hipMemcpyDeviceToHost, s));
     KC_HIP_L(hipMemcpyAsync(hq_ovf.data(), qovf, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, s));
     const hipError_t se = hipStreamSynchronize(s);
-    if (se != hipSuccess) {
-        destroy_events();
-        return fail(KC_ERR_DEVICE, "counting kernels failed: %s", hipGetErrorString(se));
-    }
+    if (se != hipSuccess) return fail(KC_ERR_DEVICE, "counting kernels failed: %s", hipGetErrorString(se));
 #undef KC_HIP_L
-    float bin_ms[NB] = {0}, total_ms = 0, prep_ms = 0;
-    hipEventElapsedTime(&prep_ms, e_start, e_cls);
-    hipEventElapsedTime(&total_ms, e_go, e_end);
+    float bin_ms[NB] = {0};
+    const float prep_ms = evs.ms(evs.start, evs.cls), total_ms = evs.ms(evs.go, evs.end);
     for (int b = 0; b < NB; ++b)
-        if (launched[b]) hipEventElapsedTime(&bin_ms[b], ev0[b], ev[b]);
-    destroy_events();
+        if (launched[b]) bin_ms[b] = evs.ms(evs.b0[b], evs.b1[b]);
     int wall_khz = 100000;
     hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ctx->dev);
     if (wall_khz <= 0) wall_khz = 100000;
@@ -2385,7 +2421,9 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
             ls->tasks += hacc[ACC * b + 2];
             ls->alg_bytes += hacc[ACC * b + 1];
             ls->tree_nodes += hacc[ACC * b + 3];
-            if (launched[b] && (dom < 0 || bin_ms[b] > bin_ms[dom])) dom = b;
+            // the dominant launch: the one whose tasks hold the waves longest
+            // (summed task durations; the side-by-side launches' spans overlap)
+            if (launched[b] && (dom < 0 || hacc[ACC * b + 5] > hacc[ACC * dom + 5])) dom = b;
             const int c = b % NCLASS;
             if (!launched[b]) continue;
             ls->class_ms[c] += bin_ms[b];
@@ -2509,9 +2547,12 @@ int kc::autotune_measured(const kc_graph *g, uint32_t k, uint32_t degree_max_out
     return KC_OK;
 }
 
-extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
+// kc_count over the tasks whose id (vertex, or oriented edge in CSR order)
+// is congruent to offset modulo stride; stride 1 counts every task.
+static int count_tasks(const kc_oriented *o, const kc_query *q, uint32_t stride, uint32_t offset, kc_result *r)
 {
     if (!o || !q || !r) return fail(KC_ERR_ARG, "kc_count: null argument");
+    if (stride == 0 || offset >= stride) return fail(KC_ERR_ARG, "task sample offset %u, stride %u", offset, stride);
     int rc = check_query(q);
     if (rc) return rc;
     hipSetDevice(o->dev);
@@ -2519,21 +2560,45 @@ extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
     std::memset(r, 0, sizeof *r);
     if (qq.k <= 2) {
         // closed forms: every vertex is a 1-clique, every edge a 2-clique
-        r->count = qq.k == 1 ? o->n : o->m;
         r->algo = qq.algo;
         r->gran = qq.gran;
         r->part_size = qq.part_size;
-        r->tasks = qq.gran == KC_GRAN_EDGE ? o->m : o->n;
+        const uint64_t ntasks = qq.gran == KC_GRAN_EDGE ? o->m : o->n;
+        const uint64_t nsample = ntasks > offset ? (ntasks - offset + stride - 1) / stride : 0;
+        r->tasks = nsample;
+        if (qq.k == 1) {
+            r->count = o->n > offset ? (o->n - offset + stride - 1) / stride : 0;
+        } else if (qq.gran == KC_GRAN_EDGE || stride == 1) {
+            r->count = stride == 1 ? o->m : nsample;
+        } else {  // out-degrees of the sampled vertices
+            std::vector<uint32_t> rp((size_t)o->n + 1);
+            if ((rc = kc_oriented_export(o, rp.data(), nullptr, nullptr))) return rc;
+            for (uint64_t u = offset; u < o->n; u += stride) r->count += rp[u + 1] - rp[u];
+        }
         return KC_OK;
     }
     kc_oriented *om = const_cast<kc_oriented *>(o);
-    std::vector<QueryRun> qs{QueryRun{o, qq}};
+    QueryRun run{o, qq};
+    run.sample = stride;
+    run.offset = offset;
+    std::vector<QueryRun> qs{run};
     std::vector<kc_result> res;
     LaunchStats ls;
     rc = run_queries(om->scr, om->stream, qs, res, &ls);
     if (rc) return rc;
     fill_result(r, res[0], ls);
     return r->overflow ? fail(KC_ERR_OVERFLOW, "k-clique count exceeds 2^64-1") : KC_OK;
+}
+
+extern "C" int kc_count(const kc_oriented *o, const kc_query *q, kc_result *r)
+{
+    return count_tasks(o, q, 1, 0, r);
+}
+
+extern "C" int kc_count_sampled(const kc_oriented *o, const kc_query *q, uint32_t stride, uint32_t offset,
+                                kc_result *r)
+{
+    return count_tasks(o, q, stride, offset, r);
 }
 
 extern "C" int kc_count_local(const kc_oriented *o, const kc_query *q, uint64_t *local, kc_result *r)

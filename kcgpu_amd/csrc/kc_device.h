@@ -37,7 +37,8 @@ struct DeviceCtx {
 // Select and check the device (gfx950).  Returns KC_OK or KC_ERR_DEVICE.
 int device_ctx(DeviceCtx **out);
 
-// Grow-only device buffer, or a view into memory owned elsewhere (owned = false).
+// Grow-only device buffer, or a view into memory owned elsewhere (owned =
+// false; ensure() on a view checks its size and never allocates).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -59,9 +60,11 @@ struct DevBuf {
 // must fit 32 bits.  in and out may alias.  tmp is scratch owned by caller.
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tmp, hipStream_t s);
 
-// scratch of one launch sequence (grow-only)
+// scratch of one launch sequence (grow-only), or views into a fixed region
+// (fixed = true: ensure() never allocates, run_queries creates no events)
 struct Scratch {
     DevBuf tasks, meta, binom, gscr, size, qdesc, local;
+    bool fixed = false;
     // what `size` holds when it was filled for one query: the oriented graph
     // (its uid), the granularity and the task count; task sizes (d+(u), or
     // |N+(u) & N+(v)|) do not depend on k, so the next query on that graph
@@ -81,7 +84,8 @@ inline uint64_t next_uid()
 struct QueryRun {
     const kc_oriented *o;
     kc_query q;
-    uint32_t sample = 1;  // count only the tasks whose id is a multiple of this (the measured autotuner)
+    uint32_t sample = 1;  // count only the tasks whose id % sample == offset (the measured autotuner,
+    uint32_t offset = 0;  // kc_count_sampled)
 };
 
 struct LaunchStats {

@@ -88,6 +88,10 @@ int device_ctx(DeviceCtx **out)
 
 int DevBuf::ensure(size_t want)
 {
+    if (!owned && p) {
+        if (want <= bytes) return KC_OK;
+        return fail(KC_ERR_LIMIT, "%zu bytes of scratch needed, the fixed region holds %zu", want, bytes);
+    }
     if (want <= bytes && p && owned) return KC_OK;
     release();
     owned = true;
