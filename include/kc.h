@@ -363,6 +363,17 @@ int kc_get_device_info(kc_device_info *info);
  * handles created after this call.  The device must be a gfx950. */
 int kc_select_device(int ordinal);
 
+/*
+ * Release the library's device state: wait for its streams and destroy them
+ * (handles carry no streams of their own).  Free every graph, oriented graph
+ * and server first; handles created before this call must not be used after
+ * it.  The next call that needs the device sets it up again.  Call it before
+ * the process exits (the Python mirror does so from an atexit handler), so
+ * that no HIP call is left for the runtime's teardown.  No reference
+ * counterpart (its streams live until exit, include/utils_cuda.cuh:150-170).
+ */
+int kc_shutdown(void);
+
 #ifdef __cplusplus
 }
 #endif

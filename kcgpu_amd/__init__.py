@@ -10,8 +10,10 @@ The names follow the reference's drop-in surface:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -140,6 +142,7 @@ SIGNATURES = {
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
     "kc_select_device": (ctypes.c_int, [ctypes.c_int]),
+    "kc_shutdown": (ctypes.c_int, []),
     "kc_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
     "kc_server_add_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p]),
     "kc_server_remove_graph": (ctypes.c_int, [_vp, ctypes.c_uint32]),
@@ -174,6 +177,28 @@ def lib():
     return _lib
 
 
+# Handles still alive at interpreter exit are freed, and the library's
+# streams destroyed, from an atexit handler: that runs before the interpreter
+# tears its objects down, so no HIP call is left for the HIP runtime's (or a
+# profiler's) own exit handlers.
+_live = weakref.WeakSet()
+_shut = False
+
+
+def shutdown():
+    """Free every live handle and release the library's device state (kc_shutdown)."""
+    global _shut
+    if _lib is None or _shut:
+        return
+    for obj in list(_live):
+        obj.close()
+    _lib.kc_shutdown()
+    _shut = True
+
+
+atexit.register(shutdown)
+
+
 def _check(rc):
     if rc != KC_OK:
         raise KCError(rc, lib().kc_last_error().decode())
@@ -189,11 +214,14 @@ class Graph:
 
     def __init__(self, handle):
         self._h = ctypes.c_void_p(handle)
+        _live.add(self)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None and not _shut:
             _lib.kc_graph_free(self._h)
-            self._h = None
+        self._h = None
+
+    __del__ = close
 
     @classmethod
     def load(cls, path):
@@ -306,11 +334,14 @@ class Oriented:
 
     def __init__(self, handle):
         self._h = ctypes.c_void_p(handle)
+        _live.add(self)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None and not _shut:
             _lib.kc_oriented_free(self._h)
-            self._h = None
+        self._h = None
+
+    __del__ = close
 
     def info(self):
         i = OrientedInfo()
@@ -407,11 +438,12 @@ class Server:
         self._h = ctypes.c_void_p()
         cfg = ServerConfig(int(budget_bytes), int(workers), int(max_batch))
         _check(lib().kc_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        _live.add(self)
 
     def close(self):
-        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+        if getattr(self, "_h", None) and self._h.value and _lib is not None and not _shut:
             _lib.kc_server_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     __del__ = close
 

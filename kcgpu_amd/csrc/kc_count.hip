@@ -36,7 +36,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kc_device.h"
@@ -52,20 +54,20 @@ constexpr int ACC = 8;  // per-class accumulator slots
 struct QDesc {
     const uint32_t *rowptr, *col, *src;
     uint32_t k, gran, sample, offset;  // tasks with id % sample == offset run
+    uint32_t psize;                    // lanes per partition of this query
 };
 
 // the graph and k of the task a wave is working on
 struct Cur {
     const uint32_t *rowptr, *col, *src;
-    uint32_t k, gran;
+    uint32_t k, gran, psize;
 };
 
 struct CountArgs {
     const QDesc *qd;
     const uint64_t *tasks;  // (query << 32) | vertex-or-edge id
     uint32_t ntasks;
-    uint32_t psize;         // lanes per partition
-    uint32_t nparts;        // partitions per wave used by the orientation DFS
+    uint32_t nparts;        // partitions per wave the orientation DFS stacks are laid out for
     uint32_t chunk;         // tasks per queue grab
     uint32_t rmax;          // row capacity of the workspace
     uint32_t wmax;          // words per row capacity
@@ -250,8 +252,8 @@ __device__ __attribute__((always_inline)) void build_rows(const Cur &g, uint32_t
 // vertices; the root is credited with the task's count by the caller):
 // partitions share lacc, so those adds are LDS atomics.
 template <bool LOCAL>
-__device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint64_t *stk, uint32_t *next_row,
-                                 uint32_t R, uint32_t W, uint32_t r, uint64_t &nodes, uint32_t &ovf,
+__device__ uint64_t orient_count(const CountArgs &a, uint32_t P, const uint64_t *rows, uint64_t *stk,
+                                 uint32_t *next_row, uint32_t R, uint32_t W, uint32_t r, uint64_t &nodes, uint32_t &ovf,
                                  uint64_t *lacc, uint32_t *path)
 {
     const int lane = lane_id();
@@ -269,7 +271,8 @@ __device__ uint64_t orient_count(const CountArgs &a, const uint64_t *rows, uint6
         }
         return cnt;
     }
-    const uint32_t P = a.psize, NP = a.nparts;
+    // P lanes per partition (the task's query); the stacks hold a.nparts partitions
+    const uint32_t NP = 64 / P < a.nparts ? 64 / P : a.nparts;
     const uint32_t part = lane / P, lip = lane % P;
     if (part >= NP) return 0;
     const uint32_t wmax = a.wmax;
@@ -728,15 +731,15 @@ __device__ uint32_t tri_lds(const uint64_t *rows, uint32_t W, const uint64_t *X,
 }
 
 template <bool LOCAL, bool WIDE = false>
-__device__ uint64_t pivot_count(const CountArgs &a, uint32_t k, const uint64_t *rows, uint64_t *stk, uint64_t *X,
-                                Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0, uint64_t &nodes,
-                                uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true,
+__device__ uint64_t pivot_count(const CountArgs &a, uint32_t P, uint32_t k, const uint64_t *rows, uint64_t *stk,
+                                uint64_t *X, Scal *scal, uint32_t *cand, uint32_t R, uint32_t W, uint32_t h0,
+                                uint64_t &nodes, uint32_t &ovf, uint64_t *lacc, uint32_t p0 = 0, bool init_x = true,
                                 uint64_t *crows2 = nullptr, uint64_t *crows4 = nullptr)
 {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
     const uint32_t wmax = a.wmax;
-    const uint32_t P = a.psize, NPp = 64 / P, part = lane / P, lip = lane % P;
+    const uint32_t NPp = 64 / P, part = lane / P, lip = lane % P;  // P lanes score one candidate
     const uint32_t lvl = LOCAL ? 4 * wmax : 2 * wmax;  // stack words per level
     uint64_t *Hc = X + wmax, *Pc = X + 2 * wmax;     // LOCAL: held / pivot sets of the node
     uint64_t acc = 0;
@@ -1515,7 +1518,7 @@ __device__ __attribute__((always_inline)) Cur select_task(const CountArgs &a, ui
         w.q = q;
     }
     const QDesc d = a.qd[q];
-    return Cur{d.rowptr, d.col, d.src, d.k, d.gran};
+    return Cur{d.rowptr, d.col, d.src, d.k, d.gran, d.psize};
 }
 
 __device__ __attribute__((always_inline)) void finish_wave(const CountArgs &a, WaveAcc &w)
@@ -1651,16 +1654,16 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
                 // (k-1)-cliques of the DAG on N+(u) / (k-2)-cliques of the DAG on T
                 build_rows<false>(g, list, offs, starts, rows, R, W, w.bytes);
                 uint32_t povf = 0;
-                uint64_t part = orient_count<LOCAL>(a, rows, stk, misc, R, W, g.gran ? k - 2 : k - 1, w.nodes, povf,
-                                                    lacc, path);
+                uint64_t part = orient_count<LOCAL>(a, g.psize, rows, stk, misc, R, W, g.gran ? k - 2 : k - 1,
+                                                    w.nodes, povf, lacc, path);
                 task_count = wave_sum64_checked(part, povf);
                 w.run_ovf |= (wave_min32(povf ? 0u : 1u) == 0) ? 1u : 0u;
                 add_checked(w.run, task_count, w.run_ovf);
             } else {
                 // SCT rooted at the node (held 1) or the edge (held 2)
                 build_rows<true>(g, list, offs, starts, rows, R, W, w.bytes);
-                task_count = pivot_count<LOCAL>(a, k, rows, stk, X, scal, offs, R, W, g.gran ? 2u : 1u, w.nodes,
-                                                w.run_ovf, lacc);
+                task_count = pivot_count<LOCAL>(a, g.psize, k, rows, stk, X, scal, offs, R, W, g.gran ? 2u : 1u,
+                                                w.nodes, w.run_ovf, lacc);
                 add_checked(w.run, task_count, w.run_ovf);
             }
             __syncthreads();
@@ -1893,8 +1896,8 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
             }
             const uint32_t h = end ? h0 : h0 + 1;
             add_checked(w.run,
-                        pivot_count<false, true>(a, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf, nullptr,
-                                                 j, false, crows2, crows4),
+                        pivot_count<false, true>(a, g.psize, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf,
+                                                 nullptr, j, false, crows2, crows4),
                         w.run_ovf);
         }
         __syncthreads();
@@ -2130,11 +2133,13 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         const kc_oriented *o = qs[i].o;
         const kc_query &q = qs[i].q;
         hq[i] = QDesc{o->rowptr.as<uint32_t>(), o->col.as<uint32_t>(), o->src.as<uint32_t>(), q.k, q.gran,
-                      std::max<uint32_t>(1, qs[i].sample), qs[i].sample > 1 ? qs[i].offset : 0};
+                      std::max<uint32_t>(1, qs[i].sample), qs[i].sample > 1 ? qs[i].offset : 0, q.part_size};
         halgo[i] = (uint8_t)(q.algo == KC_ALGO_PIVOT);
         hoff[i + 1] = hoff[i] + (q.gran == KC_GRAN_EDGE ? o->m : o->n);
         kmax = std::max(kmax, q.k);
-        if (i == 0) P = q.part_size;
+        // every query keeps its own partition size; the orientation DFS stacks
+        // are laid out for the smallest (the most partitions per wave)
+        P = i == 0 ? q.part_size : std::min(P, q.part_size);
     }
     const uint64_t total = hoff[nq];
     if (total >= (1ull << 32) * 64) return fail(KC_ERR_LIMIT, "too many tasks in one launch");
@@ -2256,7 +2261,10 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     uint64_t reg_tasks = 0;
     for (int b = 0; b < NB; ++b)
         if (b % NCLASS <= 2) reg_tasks += bcnt[b];
-    const bool fan = evs.on && grid_share == 1 && reg_tasks < FAN_MAX_TASKS_PER_CU * (uint64_t)ctx->cu;
+    // KC_SIDE_BY_SIDE=1 / =0 forces the choice (tests run both ways)
+    const char *fan_env = std::getenv("KC_SIDE_BY_SIDE");
+    const bool fan_rule = reg_tasks < FAN_MAX_TASKS_PER_CU * (uint64_t)ctx->cu;
+    const bool fan = evs.on && grid_share == 1 && (fan_env && *fan_env ? *fan_env == '1' : fan_rule);
     bool side_waits[DeviceCtx::NSIDE] = {false}, fan_recorded = false;
     hipStream_t bin_stream[NB];
     int nside = 0;
@@ -2277,7 +2285,6 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         a.qd = dq;
         a.tasks = scr.tasks.as<uint64_t>() + boff[b];
         a.ntasks = bcnt[b];
-        a.psize = P;
         a.binom = scr.binom.as<uint64_t>();
         a.binom_ovf = scr.binom.as<uint8_t>() + nbin * 8;
         a.bstride = bstride;

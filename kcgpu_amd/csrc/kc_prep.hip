@@ -772,6 +772,30 @@ int kc_select_device(int ordinal)
     return device_ctx(&ctx);
 }
 
+// Wait for and destroy the library's streams; the next call sets the device
+// up again.  Run before the process exits so that no HIP call is left for
+// the runtime's own teardown.
+int kc_shutdown(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_state == 1) {
+        hipSetDevice(g_ctx.dev);
+        if (g_ctx.stream) {
+            hipStreamSynchronize(g_ctx.stream);
+            hipStreamDestroy(g_ctx.stream);
+            g_ctx.stream = nullptr;
+        }
+        for (hipStream_t &st : g_ctx.side)
+            if (st) {
+                hipStreamSynchronize(st);
+                hipStreamDestroy(st);
+                st = nullptr;
+            }
+    }
+    g_state = 0;
+    return KC_OK;
+}
+
 int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
 {
     if (!g || !out) return fail(KC_ERR_ARG, "kc_orient: null argument");

@@ -199,6 +199,35 @@ def test_wide_kernel_near_clique_closures_vs_oracle():
                 assert r.count == exp, (mode, k, gran, r.count, exp)
 
 
+@pytest.mark.parametrize("side_by_side", ["1", "0"])
+@pytest.mark.parametrize("mode", ["degree", "degen"])
+def test_near_clique_closed_forms_k5_to_11(near_cliques, mode, side_by_side, monkeypatch):
+    """Complements of sparse graphs (triangles, K4, paths, stars, C4, paws,
+    diamonds, random 4..7-vertex graphs): the exact counts follow from the
+    complement's independence polynomial (tests/golden/make_golden.py).
+    Tasks of up to 199 vertices (register classes), 299 (wide kernel, LDS) and
+    1099 (wide kernel, HBM slices) at k = 5..11; counts of 2^64 or more must
+    come back as overflow.  The register classes run side by side and in
+    sequence."""
+    from conftest import complement_edges
+    monkeypatch.setenv("KC_SIDE_BY_SIDE", side_by_side)
+    for g in near_cliques:
+        a, b = complement_edges(g["n"], g["h_edges"])
+        Og = K.Graph.from_edges(a, b, g["n"]).orient(mode)
+        for k in range(5, 12):
+            exp = g["counts"][k]
+            for gran in GRANS:
+                if gran == "edge" and g["n"] > 1000 and (k not in (7, 8) or mode == "degree" or side_by_side == "0"):
+                    continue  # 350,000 edge tasks of 257..1099 vertices: 10 s per count; k=7 exact, k=8 overflow
+                r = Og.count(k, "pivot", gran, part_size=1, check=False)
+                print(g["name"], mode, side_by_side, k, gran, r.count, r.overflow, f"{r.raw['count_ms']:.1f} ms",
+                      r.raw["class_tasks"], flush=True)
+                if exp >= 2 ** 64:
+                    assert r.overflow, (g["name"], mode, k, gran)
+                else:
+                    assert r.count == exp and not r.overflow, (g["name"], mode, k, gran, r.count, exp)
+
+
 def test_measured_autotune_plan_counts():
     # a graph with out-degrees above 64: the plan is measured; counting with
     # it gives the oracle's count
@@ -401,6 +430,25 @@ def test_cli_end_to_end(tmp_path, golden):
         assert f"kcliques={g['counts'][k]} k={k} overflow=0" in r.stdout
         if orient == "degen":
             assert f"Max Core = {g['max_core']}" in r.stdout
+
+
+def test_cli_kcore(tmp_path, golden):
+    """-m kcore (src/main.cu:248-261): the device peel alone; prints the
+    reference's "Max Core" line (networkx core number) and counts nothing,
+    whatever -o says."""
+    import os
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kcgpu_amd", "bin", "kcgpu")
+    for name in ("planted_400", "ba_500_10", "K25", "cycle_50"):
+        g = next(x for x in golden if x["name"] == name)
+        p = tmp_path / f"{name}.txt"
+        p.write_text("".join(f"{b} {a}\n" for a, b in g["edges"]))
+        for orient in ("degen", "degree"):
+            r = subprocess.run([cli, "-g", str(p), "-m", "kcore", "-o", orient], capture_output=True, text=True,
+                               timeout=120)
+            assert r.returncode == 0, r.stderr
+            assert f"Max Core = {g['max_core']}\n" in r.stdout, (name, r.stdout)
+            assert "Preprocess time" in r.stdout and "kcliques=" not in r.stdout and "Counter" not in r.stdout
 
 
 def test_cli_graph_stats(tmp_path):

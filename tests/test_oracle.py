@@ -128,6 +128,39 @@ def test_counts_vs_golden(golden, algo, gran):
                 assert r["count"] == g["counts"][k] and not r["overflow"], (g["name"], k, algo, gran)
 
 
+def test_counts_k11_vs_golden(golden):
+    """k = 11 (the top of the counted range) from networkx"""
+    for g in golden:
+        rp, col = _csr(g)
+        orp, oc = O.orient(rp, col, O.peel_priority(rp, col)[0])
+        for algo in ("pivot", "pivot_tails"):
+            for gran in ("node", "edge"):
+                r = O.count(orp, oc, 11, algo, gran)
+                assert r["count"] == g["counts"][11] and not r["overflow"], (g["name"], algo, gran)
+
+
+def test_near_clique_closed_forms(near_cliques):
+    """The oracle on 200-vertex complements of sparse graphs at k = 8..11:
+    equal to the counts from the complement's independence polynomial."""
+    from conftest import complement_edges
+    for g in near_cliques:
+        if g["n"] > 200:
+            continue
+        a, b = complement_edges(g["n"], g["h_edges"])
+        rp, col = O.csr_from_edges(a, b, g["n"])
+        for key in (None, O.peel_priority(rp, col)[0]):
+            orp, oc = O.orient(rp, col, key)
+            for k in (8, 9, 10, 11):
+                r = O.count(orp, oc, k, "pivot_tails", "node" if k % 2 else "edge")
+                assert r["count"] == g["counts"][k] and not r["overflow"], (g["name"], k)
+    # a planted mistake: one complement edge left out changes the count
+    g = near_cliques[0]
+    a, b = complement_edges(g["n"], g["h_edges"][1:])
+    rp, col = O.csr_from_edges(a, b, g["n"])
+    orp, oc = O.orient(rp, col)
+    assert O.count(orp, oc, 8, "pivot_tails", "node")["count"] != g["counts"][8]
+
+
 def test_brute_force_vs_golden(golden):
     for g in golden:
         if g["n"] > 64:
