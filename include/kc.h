@@ -260,6 +260,23 @@ int kc_count_sampled(const kc_oriented *o, const kc_query *q, uint32_t stride, u
 int kc_count_local(const kc_oriented *o, const kc_query *q, uint64_t *local, kc_result *result);
 
 /*
+ * kc_count without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (a serving loop with a fixed
+ * HBM footprint, a harness that forbids allocation during its run).  The
+ * launch scratch (4 + 8 bytes per task, the binomial table, counters, and
+ * HBM bitmap slices for tasks above 1024 vertices) is carved out of one
+ * device array of KC_NOALLOC_SCRATCH_MIB MiB that the library's code object
+ * holds on every device it is loaded on, until the process ends.  A query
+ * whose scratch does not fit returns KC_ERR_LIMIT (for example edge tasks of
+ * a graph above about 22 M oriented edges).  The count equals kc_count's;
+ * the timing fields of the result stay 0 (no HIP events are created).  Calls
+ * on one device share the array: they are serialized by a lock inside the
+ * library, and the array keeps the last call's contents.
+ */
+#define KC_NOALLOC_SCRATCH_MIB 256
+int kc_count_noalloc(const kc_oriented *o, const kc_query *q, kc_result *result);
+
+/*
  * Parse the reference's -q grammar: [o|p]<P>[b|n|h] (include/Config.h:235-271).
  * Unlike the reference, a spec of two characters ("p8") is parsed too (the
  * reference silently ignores specs shorter than 3 characters) and an invalid

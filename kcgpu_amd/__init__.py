@@ -32,6 +32,8 @@ KC_ERR_BUDGET = 7
 KC_ERR_NOTFOUND = 8
 KC_ERR_LIMIT = 9
 
+NOALLOC_SCRATCH_MIB = 256  # KC_NOALLOC_SCRATCH_MIB: size of kc_count_noalloc's fixed device array
+
 GEN_ER, GEN_CHUNG_LU, GEN_RMAT, GEN_BA = 0, 1, 2, 3
 ORIENT = {"degree": 0, "degen": 1}
 ALGO = {"orient": 0, "pivot": 1}
@@ -139,6 +141,7 @@ SIGNATURES = {
                                         ctypes.POINTER(Result)]),
     "kc_count_local": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(Result)]),
+    "kc_count_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(Query), ctypes.POINTER(Result)]),
     "kc_parse_spec": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Query)]),
     "kc_get_device_info": (ctypes.c_int, [ctypes.POINTER(DeviceInfo)]),
     "kc_select_device": (ctypes.c_int, [ctypes.c_int]),
@@ -389,6 +392,20 @@ class Oriented:
         if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
             _check(rc)
         return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+def _count_noalloc(self, k, algo="pivot", gran="node", part_size=1, binary=True, auto_tune=False, check=True):
+    """kc_count_noalloc: the same count with no device allocation (the launch
+    scratch is the library's fixed device array; timing fields stay 0)."""
+    q = _query(k, algo, gran, part_size, binary, auto_tune)
+    r = Result()
+    rc = lib().kc_count_noalloc(self._h, ctypes.byref(q), ctypes.byref(r))
+    if rc not in (KC_OK, KC_ERR_OVERFLOW) or (check and rc != KC_OK):
+        _check(rc)
+    return CountResult(int(r.count), bool(r.overflow), r.as_dict())
+
+
+Oriented.count_noalloc = _count_noalloc
 
 
 def _count_local(self, k, algo="pivot", gran="node", part_size=8, binary=True, auto_tune=False, check=True):

@@ -2487,7 +2487,7 @@ kc_query kc::tune_query(const kc_oriented *o, const kc_query *q)
     return t;
 }
 
-static void fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls)
+void kc::fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls)
 {
     *r = res;
     r->tasks = ls.tasks;

@@ -111,6 +111,9 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
 int check_query(const kc_query *q);
 kc_query tune_query(const kc_oriented *o, const kc_query *q);
 
+// copy one query's result and the launch statistics into the ABI's kc_result
+void fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls);
+
 }  // namespace kc
 
 struct kc_oriented {

@@ -35,6 +35,12 @@ def test_library_exports_every_declared_symbol():
     assert sorted(K.SIGNATURES) == syms
 
 
+def test_noalloc_scratch_size_matches_header():
+    hdr = open(os.path.join(ROOT, "include", "kc.h")).read()
+    m = re.search(r"#define KC_NOALLOC_SCRATCH_MIB (\d+)", hdr)
+    assert m and int(m.group(1)) == K.NOALLOC_SCRATCH_MIB
+
+
 # ---- loaders -----------------------------------------------------------------
 def _edges(G):
     rp, col = G.csr()
