@@ -37,33 +37,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def splitmix64(x):
-    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-    z = x
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-    return z ^ (z >> 31)
-
-
-def batch_mix(count=4096):
-    """The batched workload: graph i has seed splitmix64(4 ^ i); even i are
-    Barabasi-Albert (n ~ U[200, 20000], attach ~ U[3, 20]), odd i Erdos-Renyi
-    (n ~ U[200, 20000], average degree ~ U[4, 40]); k ~ U{4..7}.  Parameters
-    are drawn from numpy's PCG64 seeded with the graph seed."""
-    import numpy as np
-    out = []
-    for i in range(count):
-        s = splitmix64(4 ^ i)
-        r = np.random.default_rng(s)
-        n = int(r.integers(200, 20001))
-        if i % 2 == 0:
-            spec = dict(kind="ba", n=n, attach=int(r.integers(3, 21)))
-        else:
-            spec = dict(kind="er", n=n, m=n * int(r.integers(4, 41)) // 2)
-        spec["k"] = int(r.integers(4, 8))
-        spec["seed"] = s
-        out.append(spec)
-    return out
+from kcgpu_amd.workloads import batch_mix, make_graph, serve_mix, splitmix64  # noqa: E402,F401
 
 
 def run_batch(args):
@@ -74,10 +48,7 @@ def run_batch(args):
     specs = batch_mix(args.graphs)
 
     def make(sp):
-        if sp["kind"] == "ba":
-            G = K.Graph.generate(K.GEN_BA, seed=sp["seed"], n=sp["n"], attach=sp["attach"])
-        else:
-            G = K.Graph.generate(K.GEN_ER, seed=sp["seed"], n=sp["n"], m=sp["m"])
+        G = make_graph(K, sp)
         return G, G.autotune(sp["k"])
 
     t = time.perf_counter()
@@ -121,42 +92,6 @@ def run_batch(args):
                       "tasks": st["tasks"],
                       "plans": {x: sum(1 for _, p in made if p["orient"] == x) for x in ("degree", "degen")}}}
     print(json.dumps(out), flush=True)
-
-
-def serve_mix(count, seed):
-    """Resident graphs for the serving workload: a seeded mix of the ER,
-    Chung-Lu, R-MAT and BA generators, each at most 2M edges."""
-    import numpy as np
-    r = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        t = i % 4
-        if t == 0:
-            n = int(r.integers(5000, 100001))
-            out.append(dict(kind="er", n=n, m=min(2_000_000, n * int(r.integers(4, 41)) // 2)))
-        elif t == 1:
-            n = int(r.integers(20000, 300001))
-            out.append(dict(kind="cl", n=n, m=min(2_000_000, n * int(r.integers(3, 7))), gamma=2.1,
-                            max_degree=float(r.integers(300, 2001))))
-        elif t == 2:
-            out.append(dict(kind="rmat", scale=int(r.integers(11, 16)), edge_factor=16))
-        else:
-            n = int(r.integers(1000, 50001))
-            out.append(dict(kind="ba", n=n, attach=int(r.integers(3, 21))))
-        out[-1]["seed"] = int(r.integers(1, 2**62))
-    return out
-
-
-def make_graph(K, sp):
-    if sp["kind"] == "er":
-        return K.Graph.generate(K.GEN_ER, seed=sp["seed"], n=sp["n"], m=sp["m"])
-    if sp["kind"] == "cl":
-        return K.Graph.generate(K.GEN_CHUNG_LU, seed=sp["seed"], n=sp["n"], m=sp["m"], gamma=sp["gamma"],
-                                max_degree=sp["max_degree"])
-    if sp["kind"] == "rmat":
-        return K.Graph.generate(K.GEN_RMAT, seed=sp["seed"], scale=sp["scale"], edge_factor=sp["edge_factor"],
-                                a=0.57, b=0.19, c=0.19)
-    return K.Graph.generate(K.GEN_BA, seed=sp["seed"], n=sp["n"], attach=sp["attach"])
 
 
 def run_serve(args):
