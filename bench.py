@@ -130,64 +130,86 @@ def run_serve(args):
     for i in range(args.resident):
         gids[i] = srv.add_graph(graphs[i], plans[i]["orient"])
     log(f"resident {args.resident} graphs in {time.perf_counter() - t:.1f}s", srv.stats())
-    rng = np.random.default_rng(55)
-    nq = args.queries
-    gaps = rng.exponential(1.0 / args.rate, nq)
-    pick = rng.random(nq)
-    ks = rng.integers(3, 11, nq)
-    modes = rng.integers(0, 2, nq)
-    tickets, sent = [None] * nq, [None] * nq
     live = args.resident
-    converted = 0
+    rates = [float(x) for x in (args.rates or str(args.rate)).split(",")]
 
-    def uploader():  # 64 graphs arrive during the run
+    def stream(rate, nq, with_uploads):
+        """Poisson (graph, k) queries at `rate` per second (0: all at once);
+        the 64 uploads arrive during the first stream"""
         nonlocal live
-        for i in range(args.resident, args.resident + args.uploads):
-            time.sleep(gaps.sum() / (2 * args.uploads))
-            gids[i] = srv.add_graph(graphs[i], plans[i]["orient"])
-            live = i + 1
+        rng = np.random.default_rng(55)
+        gaps = rng.exponential(1.0 / rate, nq) if rate > 0 else np.zeros(nq)
+        pick = rng.random(nq)
+        ks = rng.integers(3, 11, nq)
+        modes = rng.integers(0, 2, nq)
+        tickets, sent = [None] * nq, [None] * nq
+        converted = 0
 
-    srv.reset_stats()
-    up = threading.Thread(target=uploader)
-    t0 = time.perf_counter()
-    up.start()
-    nxt = t0
-    for i in range(nq):
-        nxt += gaps[i]
-        d = nxt - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        gi = int(pick[i] * live)
-        algo = "pivot" if modes[i] else "orient"
-        if algo == "orient" and orient_bound(gi, int(ks[i])) > ORIENT_BOUND:
-            algo = "pivot"
-            converted += 1
-        q = dict(k=int(ks[i]), algo=algo, gran=plans[gi]["gran"], part_size=1)
-        sent[i] = (gi, q)
-        tickets[i] = srv.submit(gids[gi], **q)
-        if (i + 1) % 100 == 0:
-            log(f"submitted {i + 1}/{nq}", srv.stats())
-    up.join()
-    while srv.stats()["completed"] < nq:  # drain, printing progress
-        time.sleep(2.0)
-        log("draining", srv.stats())
-    srv.drain()
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    got = [srv.wait(tk).count for tk in tickets]
-    log("serving", st)
-    # per-call plain kernels on the same queries, back to back
+        def uploader():  # graphs arriving while queries run
+            nonlocal live
+            for i in range(args.resident, args.resident + args.uploads):
+                time.sleep(max(gaps.sum(), 1.0) / (2 * args.uploads))
+                gids[i] = srv.add_graph(graphs[i], plans[i]["orient"])
+                live = i + 1
+
+        srv.reset_stats()
+        up = threading.Thread(target=uploader) if with_uploads else None
+        t0 = time.perf_counter()
+        if up:
+            up.start()
+        nxt = t0
+        for i in range(nq):
+            nxt += gaps[i]
+            d = nxt - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            gi = int(pick[i] * live)
+            algo = "pivot" if modes[i] else "orient"
+            if algo == "orient" and orient_bound(gi, int(ks[i])) > ORIENT_BOUND:
+                algo = "pivot"
+                converted += 1
+            q = dict(k=int(ks[i]), algo=algo, gran=plans[gi]["gran"], part_size=1)
+            sent[i] = (gi, q)
+            tickets[i] = srv.submit(gids[gi], **q)
+            if (i + 1) % 200 == 0:
+                log(f"rate {rate}: submitted {i + 1}/{nq}", srv.stats())
+        if up:
+            up.join()
+        while srv.stats()["completed"] < nq:  # drain, printing progress
+            time.sleep(1.0)
+            log("draining", srv.stats())
+        srv.drain()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        got = [srv.wait(tk).count for tk in tickets]
+        log(f"rate {rate}", st)
+        return sent, got, st, wall, converted
+
+    sweep = []
+    first = None
+    for ri, rate in enumerate(rates):
+        sent, got, st, wall, converted = stream(rate, args.queries, ri == 0)
+        sweep.append({"offered_rate": rate if rate > 0 else "all at once", "queries_per_s": st["queries_per_s"],
+                      "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+                      "mean_batch": st["mean_batch"], "resident_admitted": st["resident_admitted"],
+                      "joined_running": st["joined_running"], "resident_launches": st["resident_launches"],
+                      "wall_s": wall})
+        if first is None:
+            first = (sent, got, st, wall, converted)
+    sent, got, st, wall, converted = first
+    # per-call plain kernels on the first stream's queries, back to back
     ors = {}
     t = time.perf_counter()
     plain = []
     each = []
+    nq = len(sent)
     for i, (gi, q) in enumerate(sent):
         if gi not in ors:
             ors[gi] = graphs[gi].orient(plans[gi]["orient"])
         t1 = time.perf_counter()
         plain.append(ors[gi].count(**q).count)
         each.append((time.perf_counter() - t1) * 1e3)
-        if (i + 1) % 100 == 0:
+        if (i + 1) % 200 == 0:
             log(f"plain {i + 1}/{nq}")
     t_plain = time.perf_counter() - t
     same = got == plain
@@ -199,11 +221,13 @@ def run_serve(args):
            "unit": "queries/s", "n_gpus": 1, "higher_is_better": True, "dtype": "u64", "data": "synthetic",
            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
            "config": {"workload": f"{args.resident} resident graphs + {args.uploads} uploaded mid-run (ER, Chung-Lu, "
-                                  f"R-MAT, BA; <=2M edges); {nq} Poisson queries at {args.rate}/s, k~U{{3..10}}, "
+                                  f"R-MAT, BA; <=2M edges); {nq} Poisson queries per rate, k~U{{3..10}}, "
                                   "orient|pivot, arrival seed 55",
-                      "offered_rate": args.rate, "wall_s": wall, "batches": st["batches"],
+                      "offered_rate": rates[0], "wall_s": wall, "batches": st["batches"],
                       "orient_queries_sent_as_pivot": converted, "orient_bound": ORIENT_BOUND,
                       "mean_batch": st["mean_batch"], "workers": args.workers,
+                      "resident_admitted": st["resident_admitted"], "joined_running": st["joined_running"],
+                      "resident_launches": st["resident_launches"], "sweep": sweep,
                       "plain_back_to_back_s": t_plain, "plain_queries_per_s": nq / t_plain,
                       "plain_ms_p50": float(np.percentile(each, 50)), "plain_ms_p99": float(np.percentile(each, 99)),
                       "plain_ms_max": float(max(each)),
@@ -415,6 +439,7 @@ def main():
     ap.add_argument("--uploads", type=int, default=64)
     ap.add_argument("--queries", type=int, default=600)
     ap.add_argument("--rate", type=float, default=8.0)
+    ap.add_argument("--rates", default="", help="serving: comma list of offered rates (0 = all at once)")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=256)
     args = ap.parse_args()

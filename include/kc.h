@@ -295,10 +295,15 @@ typedef struct {
 /*
  * Pick orientation, granularity, algorithm and partition size for counting
  * k-cliques of g (the reference takes all four from the command line).  One
- * O(m) host pass over the CSR; when the degree order leaves out-degrees
- * above 64 and k >= 4, the orientation and granularity are measured on the
- * device (both orientations built, a 1/64 sample of the tasks counted per
- * pair), which needs a GPU and costs up to about one count.
+ * O(m) host pass over the CSR (include/Config.h:235-271 is what it replaces:
+ * -o, -p and -q).  When the degree order keeps every out-degree at or below
+ * 64, or k < 4, a rule decides (pivoting, P = 1, degree order and node tasks
+ * when every task fits one 64-bit word).  Otherwise everything is measured
+ * on the device: both orientations are built, a 1/64 sample of the tasks is
+ * counted with pivoting for each (orientation, granularity) pair, and at
+ * k <= 5 the same sample is also counted with the orientation DFS at
+ * partition sizes 1, 8 and 64 on the winning pair; the fastest of all makes
+ * the plan.  That needs a GPU and costs up to about one count.
  */
 int kc_autotune(const kc_graph *g, uint32_t k, kc_plan *plan);
 
@@ -333,19 +338,28 @@ typedef struct {
 
 typedef struct {
     uint64_t submitted, completed;  /* completed since the last kc_server_reset_stats */
-    uint64_t batches;
-    double mean_batch;              /* queries per batch */
+    uint64_t batches;               /* worker batches + admissions into the resident launch */
+    double mean_batch;              /* queries per batch or admission */
     double p50_ms, p99_ms, mean_ms; /* submit -> result latency */
     double queries_per_s;           /* completed / (last completion - first submit) */
     uint64_t graphs, resident_bytes, held_scratch_bytes;
+    uint64_t resident_admitted;     /* queries counted by the resident launch (the rest by workers) */
+    uint64_t joined_running;        /* of those, queries whose tasks were appended while it ran */
+    uint64_t resident_launches;     /* times the resident launch was (re)started */
 } kc_server_stats;
 
 /*
  * Serving loop (no reference counterpart).  Graphs stay oriented in HBM;
- * queries are queued by kc_server_submit and counted by worker threads that
- * batch every pending query that fits the memory budget into persistent
- * launches (continuous batching; batches of different workers overlap on the
- * device).  A result is exactly what kc_count returns for the query alone.
+ * kc_server_submit queues a query and returns.  Pivoting queries on graphs
+ * whose tasks all fit 64 vertices are appended, as they arrive, to the open
+ * task queues of one resident persistent launch, whose waves drain them next
+ * to the queries already running (continuous batching: a query admitted
+ * while others are on the GPU joins the running launch when a query slot
+ * and queue room are free; queries of at most 65,536 tasks use an express
+ * queue the waves try first).  Other queries are counted by worker threads
+ * that batch every pending query fitting the memory budget into persistent
+ * launches on their own streams.  A result is exactly what kc_count returns
+ * for the query alone.
  */
 int kc_server_create(const kc_server_config *cfg, kc_server **out);
 /* Orient g and keep it resident; KC_ERR_BUDGET when it does not fit. */

@@ -99,7 +99,9 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("graphs", ctypes.c_uint64),
-                ("resident_bytes", ctypes.c_uint64), ("held_scratch_bytes", ctypes.c_uint64)]
+                ("resident_bytes", ctypes.c_uint64), ("held_scratch_bytes", ctypes.c_uint64),
+                ("resident_admitted", ctypes.c_uint64), ("joined_running", ctypes.c_uint64),
+                ("resident_launches", ctypes.c_uint64)]
 
 
 class BatchStats(ctypes.Structure):

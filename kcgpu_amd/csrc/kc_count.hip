@@ -56,6 +56,7 @@ struct QDesc {
     uint32_t k, gran, sample, offset;  // tasks with id % sample == offset run
     uint32_t psize;                    // lanes per partition of this query
 };
+static_assert(sizeof(QDesc) % 8 == 0, "QDesc is read as 64-bit words");
 
 // the graph and k of the task a wave is working on
 struct Cur {
@@ -1582,6 +1583,200 @@ void k_pivot_reg(CountArgs a)
     finish_wave(a, w);
 }
 
+// ---- open queues: the serving loop's resident launches ------------------------
+// The server keeps resident launches of k_pivot_open while queries arrive,
+// each draining one FIFO queue of tasks.  A query is a segment (base, slot,
+// task count) of the queue's task index space.  The host admits a query with
+// plain stores to pinned host memory (the slot's descriptor, the segment,
+// then the queue's task total with release), so admission needs no HIP call
+// and no free CU while the launch runs.  A wave takes the next task index
+// with one fetch-add on a device word; an index beyond what the device knows
+// is published is held until it is (at most every POLL_TICKS one wave reads
+// the host's total and raises the device's copy).  The host ends a launch by
+// publishing a stop segment of one index per wave, so no index is ever
+// handed back; a wave also ends by itself when it has held an index for
+// safety_ticks (the host is gone).  A task's size is computed by the wave
+// that stages it, and tasks below the clique part still to find add 0.  A
+// wave keeps the running count of its slot and flushes it, with the number
+// of tasks it finished, when the slot changes and before it waits; the flush
+// that completes the slot's task count resets the slot's sums and writes the
+// result to the slot's mailbox in host memory.
+constexpr uint32_t OPEN_STOP = 0xFFFFFFFFu;  // slot of a stop segment
+
+struct OpenArgs {
+    CountArgs c;              // binom, qcount / qovf per slot, acc (statistics)
+    uint64_t *counter;        // next task index
+    uint64_t *known;          // task indices the device knows are published
+    uint64_t *last_poll;      // wall clock of the last read of the host's total
+    uint64_t *qdone;          // per slot: tasks finished
+    const uint64_t *htotal;   // host: task indices published
+    const uint64_t *hnseg;    // host: segments published
+    const uint64_t *hseg;     // host: per segment base, then (slot << 32) | task count
+    const OpenSlot *hslot;    // host: per slot descriptor
+    OpenMail *mail;           // host: per slot result
+    uint64_t safety_ticks;    // wall-clock ticks (100 MHz) a wave waits for its index at most
+};
+
+constexpr uint64_t POLL_TICKS = 200;  // 2 us
+
+__device__ inline uint64_t ld_agent(uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline uint64_t ld_host(const uint64_t *p)
+{
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// lane 0: read the host's task total, at most once per POLL_TICKS over the launch
+__device__ inline void open_refresh(const OpenArgs &oa)
+{
+    const uint64_t now = wall_clock64();
+    uint64_t last = ld_agent(oa.last_poll);
+    if (now - last < POLL_TICKS) return;
+    if (!__hip_atomic_compare_exchange_strong(oa.last_poll, &last, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+        return;
+    const uint64_t h = ld_host(oa.htotal);
+    if (h > ld_agent(oa.known)) __hip_atomic_fetch_max(oa.known, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the segment a wave last resolved an index in (indices a wave takes only grow)
+struct SegCache {
+    uint64_t seg = 0, base = 0, desc = 0;
+    bool valid = false;
+};
+
+// lane 0: the segment holding published index i (segments are in base
+// order; binary search in host memory from the cached one)
+__device__ inline void open_resolve(const OpenArgs &oa, SegCache &sc, uint64_t i)
+{
+    if (sc.valid && i >= sc.base && i - sc.base < (uint32_t)sc.desc) return;
+    uint64_t lo = sc.valid ? sc.seg : 0, hi = ld_host(oa.hnseg);  // base[lo] <= i < total
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (ld_host(oa.hseg + 2 * mid) <= i) lo = mid;
+        else hi = mid;
+    }
+    sc.seg = lo;
+    sc.base = ld_host(oa.hseg + 2 * lo);
+    sc.desc = ld_host(oa.hseg + 2 * lo + 1);
+    sc.valid = true;
+}
+
+// flush the running count of the current slot and the tasks finished for it;
+// the flush that completes the slot publishes its result and resets its sums
+__device__ __attribute__((always_inline)) void open_flush(const OpenArgs &oa, WaveAcc &w, uint32_t &pend,
+                                                         uint64_t total, uint32_t seq)
+{
+    if (w.q == INF32) return;
+    const CountArgs &a = oa.c;
+    if (lane_id() == 0 && pend) {
+        const uint32_t q = w.q;
+        const unsigned long long old = atomicAdd((unsigned long long *)&a.qcount[q], (unsigned long long)w.run);
+        if (old + w.run < old || w.run_ovf) atomicOr(&a.qovf[q], 1u);
+        const uint64_t before = __hip_atomic_fetch_add(&oa.qdone[q], (uint64_t)pend, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        if (before + pend == total) {
+            const uint64_t cnt = ld_agent(&a.qcount[q]);
+            const uint32_t ovf = __hip_atomic_load(&a.qovf[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.qcount[q], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.qovf[q], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&oa.qdone[q], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&oa.mail[q].count, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&oa.mail[q].ovf, ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&oa.mail[q].seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    w.any_ovf |= w.run_ovf;
+    w.run = 0;
+    w.run_ovf = 0;
+    pend = 0;
+    w.q = INF32;  // the slot may hold another query by the time this wave sees it again
+}
+
+// one-word rows: every task of the graphs the resident launches take fits
+// 64 vertices, so the kernel keeps the one-word kernel's register budget
+// (eight waves per SIMD)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_pivot_open(OpenArgs oa)
+{
+    extern __shared__ uint64_t lds[];
+    const CountArgs &a = oa.c;
+    const int lane = lane_id();
+    const Layout L = make_layout(a.rmax, a.wmax, 0, 0, true);
+    uint32_t *list = reinterpret_cast<uint32_t *>(lds + L.list);
+    uint32_t *offs = reinterpret_cast<uint32_t *>(lds + L.offs);
+    uint32_t *starts = reinterpret_cast<uint32_t *>(lds + L.starts);
+    uint64_t *rows = lds + L.rows;
+    WaveAcc w;
+    SegCache sc;
+    uint32_t pend = 0, seq = 0;
+    uint64_t total = 0;
+    Cur g{};
+    for (;;) {
+        // lane 0 takes an index and waits until it is published
+        uint64_t idx = 0;
+        uint32_t ready = 0;
+        if (lane == 0) {
+            idx = __hip_atomic_fetch_add(oa.counter, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            open_refresh(oa);
+            ready = idx < ld_agent(oa.known);
+        }
+        ready = uni32(__shfl(ready, 0, 64));
+        if (!ready) {
+            open_flush(oa, w, pend, total, seq);  // nothing held back while the wave waits
+            if (lane == 0) {
+                const uint64_t t0 = wall_clock64();
+                while (!ready && wall_clock64() - t0 < oa.safety_ticks) {
+                    __builtin_amdgcn_s_sleep(127);
+                    open_refresh(oa);
+                    ready = idx < ld_agent(oa.known);
+                }
+            }
+            ready = uni32(__shfl(ready, 0, 64));
+            if (!ready) break;  // the host is gone: give up the index
+        }
+        uint32_t slot = 0, task = 0;
+        if (lane == 0) {
+            open_resolve(oa, sc, idx);
+            slot = (uint32_t)(sc.desc >> 32);
+            task = (uint32_t)(idx - sc.base);
+        }
+        slot = uni32(__shfl(slot, 0, 64));
+        task = uni32(__shfl(task, 0, 64));
+        if (slot == OPEN_STOP) break;
+        if (slot != w.q) {
+            open_flush(oa, w, pend, total, seq);
+            w.q = slot;
+            // the slot's descriptor, from host memory (lanes 0..5 read one word each)
+            const uint64_t *hs = reinterpret_cast<const uint64_t *>(oa.hslot + slot);
+            const uint64_t word = lane < (int)(sizeof(OpenSlot) / 8) ? ld_host(hs + lane) : 0ull;
+            OpenSlot d;
+            uint64_t *dw = reinterpret_cast<uint64_t *>(&d);
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(OpenSlot) / 8); ++i) dw[i] = uni64(rdlane64(word, (uint32_t)i));
+            g = Cur{d.rowptr, d.col, d.src, d.k, d.gran, 1u};
+            total = d.total;
+            seq = d.seq;
+        }
+        ++w.ntask;
+        ++pend;
+        const uint32_t R = stage_task(g, task, list, w.bytes);
+        const uint32_t need = g.gran ? g.k - 2 : g.k - 1;
+        if (R >= need) {
+            if (g.gran && g.k == 3) {
+                add_checked(w.run, R, w.run_ovf);
+            } else {
+                __syncthreads();
+                build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
+                add_checked(w.run,
+                            pivot_reg<1>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf, 0u, nullptr),
+                            w.run_ovf);
+            }
+        }
+        __syncthreads();
+    }
+    open_flush(oa, w, pend, total, seq);
+    finish_wave(a, w);
+}
+
 // ---- the generic kernel: LDS (or HBM) bitmap, orientation or pivoting ------
 // LOCAL: per-vertex counts of one query into a.local (the -m lkc path): each
 // task zeroes its vertex accumulators, the traversal fills them, and at the
@@ -2449,6 +2644,188 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     return KC_OK;
 }
 
+
+// ---- OpenEngine (host) --------------------------------------------------------
+OpenEngine::~OpenEngine()
+{
+    if (dev >= 0) hipSetDevice(dev);
+    for (int j = 0; j < NQ; ++j) {
+        if (running(j) && !qs[j].stopped) stop(j);
+        if (qs[j].run) hipStreamSynchronize(qs[j].run);
+        if (qs[j].done) hipEventDestroy(qs[j].done);
+        if (qs[j].run) hipStreamDestroy(qs[j].run);
+    }
+    if (adm) {
+        hipStreamSynchronize(adm);
+        hipStreamDestroy(adm);
+    }
+    if (pinned) hipHostFree(pinned);
+}
+
+int OpenEngine::init(int device)
+{
+    dev = device;
+    KC_HIP(hipSetDevice(dev));
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    KC_HIP(hipStreamCreate(&adm));
+    for (Queue &q : qs) {
+        KC_HIP(hipStreamCreate(&q.run));
+        KC_HIP(hipEventCreateWithFlags(&q.done, hipEventDisableTiming));
+    }
+    if ((rc = qcount.ensure(8 * SLOTS)) || (rc = qovf.ensure(4 * SLOTS)) || (rc = qdone.ensure(8 * SLOTS)) ||
+        (rc = ctl.ensure(8 * CTL_WORDS * NQ)) || (rc = acc.ensure(8 * ACC)))
+        return rc;
+    const size_t qbytes = 16 + 16 * NSEG;  // total, nseg, then (base, slot|count) per segment
+    const size_t bytes = NQ * qbytes + sizeof(OpenSlot) * SLOTS + sizeof(OpenMail) * SLOTS;
+    KC_HIP(hipHostMalloc((void **)&pinned, bytes, hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(pinned, 0, bytes);
+    KC_HIP(hipHostGetDevicePointer((void **)&pinned_dev, pinned, 0));
+    for (int j = 0; j < NQ; ++j) {
+        uint64_t *h = reinterpret_cast<uint64_t *>(pinned + j * qbytes);
+        qs[j].htotal = h;
+        qs[j].hnseg = h + 1;
+        qs[j].hseg = h + 2;
+    }
+    hslot = reinterpret_cast<OpenSlot *>(pinned + NQ * qbytes);
+    mail = reinterpret_cast<OpenMail *>(hslot + SLOTS);
+    // C(n, r) for n <= OPEN_MAX_ROWS + 1, r <= KC_MAX_K: every query a launch can take
+    {
+        std::vector<uint64_t> bv;
+        std::vector<uint8_t> bo;
+        binom_table(OPEN_MAX_ROWS + 1, KC_MAX_K, bv, bo);
+        const size_t nbin = (size_t)(OPEN_MAX_ROWS + 2) * (KC_MAX_K + 1);
+        if ((rc = binom.ensure(nbin * 9 + 16))) return rc;
+        KC_HIP(hipMemcpyAsync(binom.p, bv.data(), nbin * 8, hipMemcpyHostToDevice, adm));
+        KC_HIP(hipMemcpyAsync(binom.as<uint8_t>() + nbin * 8, bo.data(), nbin, hipMemcpyHostToDevice, adm));
+        KC_HIP(hipStreamSynchronize(adm));
+    }
+    // per-slot sums start at zero; the flush that completes a slot zeroes them again
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, adm, qcount.as<uint32_t>(), (uint64_t)2 * SLOTS, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, adm, qovf.as<uint32_t>(), (uint64_t)SLOTS, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, adm, qdone.as<uint32_t>(), (uint64_t)2 * SLOTS, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, adm, ctl.as<uint32_t>(), (uint64_t)2 * CTL_WORDS * NQ, 0u);
+    KC_HIP(hipGetLastError());
+    KC_HIP(hipStreamSynchronize(adm));
+    const Layout L = make_layout(OPEN_MAX_ROWS, OPEN_MAX_ROWS / 64, 0, 0, true);
+    lds = L.total * 8;
+    KC_HIP(hipFuncSetAttribute((const void *)k_pivot_open, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pivot_open, 64, lds));
+    if (per_cu < 12) return fail(KC_ERR_DEVICE, "resident launches: %d waves per CU", per_cu);
+    // express: 4 waves per CU; bulk: the rest but 4, which stay free for the
+    // workers' launches
+    qs[0].blocks = (uint32_t)(4 * ctx->cu);
+    qs[1].blocks = (uint32_t)((per_cu - 8) * ctx->cu);
+    free_slots.clear();
+    for (uint32_t i = SLOTS; i-- > 0;) free_slots.push_back(i);
+    return KC_OK;
+}
+
+bool OpenEngine::running(int j)
+{
+    Queue &q = qs[j];
+    if (!q.launched) return false;
+    if (hipEventQuery(q.done) == hipSuccess) q.launched = false;
+    return q.launched;
+}
+
+int OpenEngine::reset(int j)
+{
+    if (running(j) || qs[j].inflight) return fail(KC_ERR_ARG, "open queue reset while in use");
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, adm, ctl.as<uint32_t>() + 2 * CTL_WORDS * j,
+                       (uint64_t)2 * CTL_WORDS, 0u);
+    KC_HIP(hipGetLastError());
+    KC_HIP(hipStreamSynchronize(adm));
+    Queue &q = qs[j];
+    __atomic_store_n(q.hnseg, 0ull, __ATOMIC_RELEASE);
+    __atomic_store_n(q.htotal, 0ull, __ATOMIC_RELEASE);
+    q.total = q.nseg = 0;
+    return KC_OK;
+}
+
+static void publish_segment(OpenEngine::Queue &q, uint32_t slot, uint64_t count)
+{
+    q.hseg[2 * q.nseg] = q.total;
+    q.hseg[2 * q.nseg + 1] = ((uint64_t)slot << 32) | count;
+    q.nseg++;
+    q.total += count;
+}
+
+// segments and descriptors are written before the counts that publish them;
+// the segment count before the task total (a wave that sees the total finds
+// its segment)
+static void publish_counts(OpenEngine::Queue &q)
+{
+    __atomic_store_n(q.hnseg, q.nseg, __ATOMIC_RELEASE);
+    __atomic_store_n(q.htotal, q.total, __ATOMIC_RELEASE);
+}
+
+void OpenEngine::admit(int j, const std::vector<OpenQueryIn> &in, std::vector<uint32_t> &slot)
+{
+    Queue &qq = qs[j];
+    slot.assign(in.size(), ~0u);
+    bool any = false;
+    for (size_t i = 0; i < in.size(); ++i) {
+        const OpenQueryIn &x = in[i];
+        const uint64_t nt = x.q.gran == KC_GRAN_EDGE ? x.o->m : x.o->n;
+        if (nt == 0) continue;
+        const uint32_t sl = free_slots.back();
+        free_slots.pop_back();
+        slot[i] = sl;
+        hslot[sl] = OpenSlot{x.o->rowptr.as<uint32_t>(), x.o->col.as<uint32_t>(), x.o->src.as<uint32_t>(), x.q.k,
+                             x.q.gran, nt, x.seq, 0};
+        publish_segment(qq, sl, nt);
+        qq.inflight++;
+        any = true;
+    }
+    if (any) publish_counts(qq);
+}
+
+void OpenEngine::stop(int j)
+{
+    Queue &q = qs[j];
+    publish_segment(q, OPEN_STOP, q.blocks);
+    publish_counts(q);
+    q.stopped = true;
+}
+
+int OpenEngine::launch(int j)
+{
+    Queue &q = qs[j];
+    OpenArgs oa{};
+    CountArgs &a = oa.c;
+    a.rmax = OPEN_MAX_ROWS;
+    a.wmax = OPEN_MAX_ROWS / 64;
+    a.binom = binom.as<uint64_t>();
+    a.binom_ovf = binom.as<uint8_t>() + (size_t)(OPEN_MAX_ROWS + 2) * (KC_MAX_K + 1) * 8;
+    a.bstride = KC_MAX_K + 1;
+    a.acc = acc.as<uint64_t>();
+    a.qcount = qcount.as<uint64_t>();
+    a.qovf = qovf.as<uint32_t>();
+    a.chunk = 1;
+    uint64_t *c = ctl.as<uint64_t>() + CTL_WORDS * j;
+    oa.counter = c;
+    oa.known = c + 1;
+    oa.last_poll = c + 2;
+    oa.qdone = qdone.as<uint64_t>();
+    auto dev_of = [&](const void *h) { return pinned_dev + (reinterpret_cast<const uint8_t *>(h) - pinned); };
+    oa.htotal = reinterpret_cast<const uint64_t *>(dev_of(q.htotal));
+    oa.hnseg = reinterpret_cast<const uint64_t *>(dev_of(q.hnseg));
+    oa.hseg = reinterpret_cast<const uint64_t *>(dev_of(q.hseg));
+    oa.hslot = reinterpret_cast<const OpenSlot *>(dev_of(hslot));
+    oa.mail = reinterpret_cast<OpenMail *>(dev_of(mail));
+    // a backstop only (the host ends every launch with a stop segment): 10 min
+    oa.safety_ticks = 60000000000ull;
+    void *args[] = {&oa};
+    KC_HIP(hipLaunchKernel((const void *)k_pivot_open, dim3(q.blocks), dim3(64), args, lds, q.run));
+    KC_HIP(hipEventRecord(q.done, q.run));
+    q.launched = true;
+    q.stopped = false;
+    return KC_OK;
+}
+
 }  // namespace kc
 
 using namespace kc;
@@ -2510,47 +2887,59 @@ void kc::fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls)
 // ---- measured autotuner ---------------------------------------------------
 // Both orientations are built and each (orientation, granularity) pair counts
 // every 64th task at this k with pivoting; the pair with the least device
-// time (classification + count of the sample) is the plan.  (Summed task
-// times, tail-free, ranked the plans worse: a sample of few large tasks runs
-// at another occupancy than one of many small ones.)
+// time (classification + count of the sample) wins.  At k <= 5, where the
+// orientation DFS can compete (its tree is at most three levels deep below
+// the task), the same sample is then counted with the orientation DFS at
+// partitions of 1, 8 and 64 lanes on the winning pair, and the algorithm and
+// partition size with the least time make the plan.  (Summed task times,
+// tail-free, ranked the plans worse: a sample of few large tasks runs at
+// another occupancy than one of many small ones.)
 int kc::autotune_measured(const kc_graph *g, uint32_t k, uint32_t degree_max_out, kc_plan *plan)
 {
     constexpr uint32_t SAMPLE = 64;
+    constexpr uint32_t ORIENT_DFS_MAX_K = 5;
     if (k > KC_MAX_K) return fail(KC_ERR_ARG, "k=%u outside 1..%d", k, KC_MAX_K);
     kc_oriented *o[2] = {nullptr, nullptr};
     int rc = kc_orient(g, KC_ORIENT_DEGREE, &o[0]);
     if (rc == KC_OK) rc = kc_orient(g, KC_ORIENT_DEGEN, &o[1]);
     double best = 0;
-    int best_o = KC_ORIENT_DEGEN;
-    uint32_t best_g = KC_GRAN_EDGE;
+    int best_oi = 1;
+    kc_query best_q{k, KC_ALGO_PIVOT, KC_GRAN_EDGE, 1, 1, 0};
     bool have = false;
-    for (int oi = 0; oi < 2 && rc == KC_OK; ++oi) {
-        for (uint32_t gran = KC_GRAN_NODE; gran <= KC_GRAN_EDGE && rc == KC_OK; ++gran) {
-            QueryRun r;
-            r.o = o[oi];
-            r.q = kc_query{k, KC_ALGO_PIVOT, gran, 1, 1, 0};
-            r.sample = SAMPLE;
-            std::vector<kc_result> res;
-            LaunchStats ls;
-            hipSetDevice(o[oi]->dev);
-            rc = run_queries(o[oi]->scr, o[oi]->stream, {r}, res, &ls);
-            const double t = ls.count_ms + ls.prep_ms;
-            if (rc == KC_OK && (!have || t < best)) {
-                have = true;
-                best = t;
-                best_o = oi == 0 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
-                best_g = gran;
-            }
+    auto measure = [&](int oi, const kc_query &q) {
+        QueryRun r;
+        r.o = o[oi];
+        r.q = q;
+        r.sample = SAMPLE;
+        std::vector<kc_result> res;
+        LaunchStats ls;
+        hipSetDevice(o[oi]->dev);
+        rc = run_queries(o[oi]->scr, o[oi]->stream, {r}, res, &ls);
+        const double t = ls.count_ms + ls.prep_ms;
+        if (rc == KC_OK && (!have || t < best)) {
+            have = true;
+            best = t;
+            best_oi = oi;
+            best_q = q;
         }
+    };
+    for (int oi = 0; oi < 2 && rc == KC_OK; ++oi)
+        for (uint32_t gran = KC_GRAN_NODE; gran <= KC_GRAN_EDGE && rc == KC_OK; ++gran)
+            measure(oi, kc_query{k, KC_ALGO_PIVOT, gran, 1, 1, 0});
+    if (k <= ORIENT_DFS_MAX_K && rc == KC_OK) {
+        const int oi = best_oi;
+        const uint32_t gran = best_q.gran;
+        for (uint32_t P : {1u, 8u, 64u})
+            if (rc == KC_OK) measure(oi, kc_query{k, KC_ALGO_ORIENT, gran, P, 1, 0});
     }
     for (kc_oriented *x : o)
         if (x) kc_oriented_free(x);
     if (rc != KC_OK) return rc;
     plan->degree_max_out = degree_max_out;
-    plan->orient_mode = best_o;
-    plan->algo = KC_ALGO_PIVOT;
-    plan->gran = best_g;
-    plan->part_size = 1;
+    plan->orient_mode = best_oi == 0 ? KC_ORIENT_DEGREE : KC_ORIENT_DEGEN;
+    plan->algo = best_q.algo;
+    plan->gran = best_q.gran;
+    plan->part_size = best_q.part_size;
     return KC_OK;
 }
 

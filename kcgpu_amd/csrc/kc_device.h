@@ -111,6 +111,78 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
 int check_query(const kc_query *q);
 kc_query tune_query(const kc_oriented *o, const kc_query *q);
 
+// ---- the serving loop's resident launch (open queues) ----------------------
+// mailbox of a query slot, in host memory: the device writes count and ovf,
+// then seq (the ticket) with release; the host waits for seq
+struct OpenMail {
+    uint64_t count;
+    uint32_t ovf;
+    uint32_t seq;
+};
+// descriptor of a query slot, in host memory, written before the query's
+// segment is published
+struct OpenSlot {
+    const uint32_t *rowptr, *col, *src;
+    uint32_t k, gran;
+    uint64_t total;  // tasks (vertices or oriented edges)
+    uint32_t seq, pad;
+};
+static_assert(sizeof(OpenSlot) == 48, "OpenSlot is read as six 64-bit words");
+constexpr uint32_t OPEN_MAX_ROWS = 64;  // largest task the resident launches take (one-word rows)
+
+struct OpenQueryIn {
+    const kc_oriented *o;
+    kc_query q;      // pivoting, k >= 3; every task of o at most OPEN_MAX_ROWS vertices
+    uint32_t seq;    // ticket, non-zero
+};
+
+// Resident launches of the pivoting kernel for tasks of up to OPEN_MAX_ROWS
+// vertices, each draining one queue of query segments that the host
+// publishes in pinned memory while it runs: queue 0 (express) for queries
+// with few tasks, on a few waves per CU, and queue 1 (bulk) on the rest.
+// Not thread-safe: one host thread (the server's engine) drives it.
+struct OpenEngine {
+    static constexpr uint32_t SLOTS = 4096;
+    static constexpr uint64_t NSEG = 1u << 16;  // segments per queue between restarts
+    static constexpr int NQ = 2;
+    static constexpr int CTL_WORDS = 32;  // device words per queue (counter, known, last poll): own 256 bytes
+    struct Queue {
+        hipStream_t run = nullptr;
+        hipEvent_t done = nullptr;   // recorded after each launch
+        bool launched = false, stopped = false;
+        uint64_t total = 0, nseg = 0;  // task indices and segments published
+        uint64_t *htotal = nullptr, *hnseg = nullptr, *hseg = nullptr;  // host (pinned)
+        uint32_t blocks = 0;
+        uint64_t inflight = 0;
+    };
+    int dev = -1;
+    hipStream_t adm = nullptr;       // resets
+    Queue qs[NQ];
+    DevBuf qcount, qovf, qdone, ctl, binom, acc;
+    uint8_t *pinned = nullptr;       // host: queues, slots, mailboxes
+    uint8_t *pinned_dev = nullptr;   // the same memory's device address
+    OpenSlot *hslot = nullptr;
+    OpenMail *mail = nullptr;
+    size_t lds = 0;
+    std::vector<uint32_t> free_slots;
+    ~OpenEngine();
+    int init(int dev);
+    // true while queue j's launch has not ended
+    bool running(int j);
+    // queue j has no launch and nothing in flight: restart its indices at 0
+    int reset(int j);
+    // publish the queries as segments of queue j; slot[i] gets query i's slot
+    // (~0u when it has no task: its count is 0).  The caller checks room()
+    // and free_slots first.
+    void admit(int j, const std::vector<OpenQueryIn> &in, std::vector<uint32_t> &slot);
+    void release(uint32_t slot) { free_slots.push_back(slot); }
+    // start queue j's launch (call when it is not running)
+    int launch(int j);
+    // end queue j's launch: one stop index per wave
+    void stop(int j);
+    uint64_t room(int j) const { return NSEG - 1 - qs[j].nseg; }  // (one segment kept for the stop)
+};
+
 // copy one query's result and the launch statistics into the ABI's kc_result
 void fill_result(kc_result *r, const kc_result &res, const LaunchStats &ls);
 

@@ -202,10 +202,13 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(const uint32_t *in, uint3
     if (blockIdx.x == np - 1 && threadIdx.x == 0) out[n] = (uint32_t)part[np];
 }
 
+__global__ void k_fill_u32(uint32_t *a, uint32_t n, uint32_t v);
+
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, DevBuf &tmp, hipStream_t s)
 {
-    if (n == 0) {
-        KC_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+    if (n == 0) {  // a fill kernel, not a memset node (the caller may be capturing a graph)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, out, 1u, 0u);
+        KC_HIP(hipGetLastError());
         return KC_OK;
     }
     uint32_t np = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -658,6 +661,36 @@ __global__ void k_peel_init(uint32_t n, const uint32_t *rowptr, int *deg, uint32
     }
 }
 
+// A stream's launches captured once into a hipGraph and replayed.  When the
+// runtime refuses the capture the same launches are enqueued on the stream
+// each time instead (same work, one launch each).
+struct Captured {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~Captured()
+    {
+        if (x) hipGraphExecDestroy(x);
+        if (g) hipGraphDestroy(g);
+    }
+    template <class F>
+    void capture(hipStream_t s, F &&enqueue)
+    {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            enqueue();
+            const hipError_t ce = hipStreamEndCapture(s, &g);
+            if (ce != hipSuccess || !g || hipGraphInstantiateWithFlags(&x, g, 0) != hipSuccess) x = nullptr;
+        }
+        (void)hipGetLastError();
+    }
+    template <class F>
+    hipError_t launch(hipStream_t s, F &&enqueue)
+    {
+        if (x) return hipGraphLaunch(x, s);
+        enqueue();
+        return hipGetLastError();
+    }
+};
+
 static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uint32_t maxdeg, const uint32_t *rowptr,
                     const uint32_t *col, uint32_t *prio, uint32_t *max_core)
 {
@@ -667,10 +700,10 @@ static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uin
         (rc = q0.ensure(sizeof(uint32_t) * (size_t)n + 4)) || (rc = q1.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
         (rc = cnt.ensure(sizeof(uint32_t) * 4)))
         return rc;
-    KC_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t) * 4, s));
     int gi = (int)std::min<uint64_t>(((uint64_t)n + 255) / 256, (uint64_t)ctx->cu * 8);
     if (gi < 1) gi = 1;
     hipLaunchKernelGGL(k_peel_init, dim3(gi), dim3(256), 0, s, n, rowptr, deg.as<int>(), prio, st.as<uint8_t>(), nnz);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, cnt.as<uint32_t>(), 4u, 0u);  // counters, max core, error
     KC_HIP(hipGetLastError());
     int per_cu = 0;
     KC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_peel, 256, 0));
@@ -699,25 +732,37 @@ static int run_peel(DeviceCtx *ctx, hipStream_t s, uint32_t n, uint32_t nnz, uin
         coop = false;
     }
     if (!coop) {
+        // Steps of five launches (gather, its control, assign, decrement, the
+        // round's control), captured once as a graph of PEEL_STEPS steps and
+        // replayed until the control kernel reports the end: one graph launch
+        // and one 4-byte read per PEEL_STEPS peel rounds or levels.  Steps past
+        // the end return at once.
+        constexpr int PEEL_STEPS = 64;
         DevBuf ctlb;
         if ((rc = ctlb.ensure(sizeof(PeelCtl)))) return rc;
         PeelCtl *ctl = ctlb.as<PeelCtl>();
         hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, ctl, a.cnt, n);
+        KC_HIP(hipGetLastError());
         const int gs = std::max(1, std::min(gi, ctx->cu * 4));
-        // every step peels a vertex or moves up a level: n + maxdeg + 2 steps end it
-        const uint64_t max_steps = (uint64_t)n + maxdeg + 2;
-        uint64_t steps = 0;
-        uint32_t phase = 0;
-        while (phase != 2) {
-            if (steps > max_steps) return fail(KC_ERR_DEVICE, "peel did not converge (internal error)");
-            for (int i = 0; i < 32; ++i, ++steps) {
+        auto steps_enqueue = [&]() {
+            for (int i = 0; i < PEEL_STEPS; ++i) {
                 hipLaunchKernelGGL(k_peel_gather, dim3(gs), dim3(256), 0, s, a, ctl);
                 hipLaunchKernelGGL(k_peel_ctl_gathered, dim3(1), dim3(1), 0, s, ctl, a.cnt, maxdeg);
                 hipLaunchKernelGGL(k_peel_assign, dim3(gs), dim3(256), 0, s, a, ctl);
                 hipLaunchKernelGGL(k_peel_decrement, dim3(gs), dim3(256), 0, s, a, ctl);
                 hipLaunchKernelGGL(k_peel_ctl_round, dim3(1), dim3(1), 0, s, ctl, a.cnt, maxdeg);
             }
-            KC_HIP(hipGetLastError());
+        };
+        Captured steps_graph;
+        steps_graph.capture(s, steps_enqueue);
+        // every step peels a vertex or moves up a level: n + maxdeg + 2 steps end it
+        const uint64_t max_steps = (uint64_t)n + maxdeg + 2;
+        uint64_t steps = 0;
+        uint32_t phase = 0;
+        while (phase != 2) {
+            if (steps > max_steps) return fail(KC_ERR_DEVICE, "peel did not converge (internal error)");
+            KC_HIP(steps_graph.launch(s, steps_enqueue));
+            steps += PEEL_STEPS;
             KC_HIP(hipMemcpyAsync(&phase, &ctl->phase, sizeof phase, hipMemcpyDeviceToHost, s));
             KC_HIP(hipStreamSynchronize(s));
         }
@@ -806,102 +851,94 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     if (rc) return rc;
     const uint32_t n = g->csr.n;
     const uint32_t nnz = (uint32_t)g->csr.col.size();
-    kc_oriented *o = new kc_oriented;
+    // the CSR is simple and symmetric, so every undirected edge is kept once
+    const uint32_t m_expect = nnz / 2;
+    std::unique_ptr<kc_oriented> o(new kc_oriented);
     o->dev = ctx->dev;
     o->n = n;
     o->mode = mode;
-    auto bail = [&](int code) {
-        delete o;
-        return code;
-    };
     o->stream = ctx->stream;
     hipStream_t s = o->stream;
-    DevBuf frow, fcol, outdeg, tmp;
+    // every buffer is sized up front, so the launches need no host read in
+    // between and run as captured graphs
+    DevBuf frow, fcol, outdeg, tmp, mx;
+    const size_t np = ((size_t)n + 2047) / 2048;
     if ((rc = frow.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = fcol.ensure(sizeof(uint32_t) * (size_t)nnz + 4)) ||
         (rc = outdeg.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = o->key.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
-        (rc = o->rowptr.ensure(sizeof(uint32_t) * ((size_t)n + 1))))
-        return bail(rc);
+        (rc = o->rowptr.ensure(sizeof(uint32_t) * ((size_t)n + 1))) ||
+        (rc = o->col.ensure(sizeof(uint32_t) * (size_t)m_expect + 4)) ||
+        (rc = o->src.ensure(sizeof(uint32_t) * (size_t)m_expect + 4)) || (rc = tmp.ensure(sizeof(uint64_t) * (np + 1))) ||
+        (rc = mx.ensure(sizeof(uint32_t))))
+        return rc;
     if (hipMemcpyAsync(frow.p, g->csr.rowptr.data(), sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         (nnz && hipMemcpyAsync(fcol.p, g->csr.col.data(), sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice, s) !=
                     hipSuccess))
-        return bail(fail(KC_ERR_DEVICE, "graph upload failed"));
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, s);
-    uint32_t max_core = 0;
-    if (mode == KC_ORIENT_DEGEN && n > 0) {
-        rc = run_peel(ctx, s, n, nnz, g->max_degree, frow.as<uint32_t>(), fcol.as<uint32_t>(), o->key.as<uint32_t>(),
-                      &max_core);
-        if (rc) {
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
-            return bail(rc);
+        return fail(KC_ERR_DEVICE, "graph upload failed");
+    struct Events {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events()
+        {
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
         }
-    }
+    } ev;
+    KC_HIP(hipEventCreate(&ev.e0));
+    KC_HIP(hipEventCreate(&ev.e1));
+    KC_HIP(hipEventRecord(ev.e0, s));
+    uint32_t max_core = 0;
+    if (mode == KC_ORIENT_DEGEN && n > 0 &&
+        (rc = run_peel(ctx, s, n, nnz, g->max_degree, frow.as<uint32_t>(), fcol.as<uint32_t>(), o->key.as<uint32_t>(),
+                       &max_core)))
+        return rc;
+    // keep flags, per-row counts, scan, stable compaction, max out-degree:
+    // one captured graph
     int grid = (int)std::min<uint64_t>(((uint64_t)n + 3) / 4, (uint64_t)ctx->cu * 16);
     if (grid < 1) grid = 1;
-    if (n > 0) {
-        if (mode == KC_ORIENT_DEGEN)
-            hipLaunchKernelGGL(k_orient_count<true>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
-                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), outdeg.as<uint32_t>());
-        else
-            hipLaunchKernelGGL(k_orient_count<false>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
-                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), outdeg.as<uint32_t>());
-    }
-    if ((rc = scan_exclusive_u32(outdeg.as<uint32_t>(), o->rowptr.as<uint32_t>(), n, tmp, s))) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        return bail(rc);
-    }
-    uint32_t m = 0;
-    hipMemcpyAsync(&m, o->rowptr.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        return bail(fail(KC_ERR_DEVICE, "orientation failed: %s", hipGetErrorString(hipGetLastError())));
-    }
-    o->m = m;
-    if ((rc = o->col.ensure(sizeof(uint32_t) * (size_t)m + 4)) || (rc = o->src.ensure(sizeof(uint32_t) * (size_t)m + 4))) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        return bail(rc);
-    }
-    if (n > 0) {
-        if (mode == KC_ORIENT_DEGEN)
-            hipLaunchKernelGGL(k_orient_write<true>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
-                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), o->rowptr.as<uint32_t>(),
-                               o->col.as<uint32_t>(), o->src.as<uint32_t>());
-        else
-            hipLaunchKernelGGL(k_orient_write<false>, dim3(grid), dim3(256), 0, s, n, frow.as<uint32_t>(),
-                               fcol.as<uint32_t>(), o->key.as<uint32_t>(), o->rowptr.as<uint32_t>(),
-                               o->col.as<uint32_t>(), o->src.as<uint32_t>());
-    }
-    // max out-degree
-    DevBuf mx;
-    if ((rc = mx.ensure(sizeof(uint32_t)))) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        return bail(rc);
-    }
-    hipMemsetAsync(mx.p, 0, sizeof(uint32_t), s);
-    if (n > 0)
-        hipLaunchKernelGGL(k_max_u32, dim3(std::max(1, std::min(grid, 1024))), dim3(256), 0, s, outdeg.as<uint32_t>(),
-                           n, mx.as<uint32_t>());
-    hipEventRecord(e1, s);
-    uint32_t maxout = 0;
-    hipMemcpyAsync(&maxout, mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-    hipError_t se = hipStreamSynchronize(s);
+    int scan_rc = KC_OK;
+    auto enqueue = [&]() {
+        const uint32_t *fr = frow.as<uint32_t>(), *fc = fcol.as<uint32_t>();
+        uint32_t *key = o->key.as<uint32_t>(), *od = outdeg.as<uint32_t>(), *orow = o->rowptr.as<uint32_t>();
+        if (n > 0) {
+            if (mode == KC_ORIENT_DEGEN)
+                hipLaunchKernelGGL(k_orient_count<true>, dim3(grid), dim3(256), 0, s, n, fr, fc, key, od);
+            else
+                hipLaunchKernelGGL(k_orient_count<false>, dim3(grid), dim3(256), 0, s, n, fr, fc, key, od);
+        }
+        scan_rc = scan_exclusive_u32(od, orow, n, tmp, s);
+        if (n > 0) {
+            if (mode == KC_ORIENT_DEGEN)
+                hipLaunchKernelGGL(k_orient_write<true>, dim3(grid), dim3(256), 0, s, n, fr, fc, key, orow,
+                                   o->col.as<uint32_t>(), o->src.as<uint32_t>());
+            else
+                hipLaunchKernelGGL(k_orient_write<false>, dim3(grid), dim3(256), 0, s, n, fr, fc, key, orow,
+                                   o->col.as<uint32_t>(), o->src.as<uint32_t>());
+        }
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, mx.as<uint32_t>(), 1u, 0u);
+        if (n > 0)
+            hipLaunchKernelGGL(k_max_u32, dim3(std::max(1, std::min(grid, 1024))), dim3(256), 0, s, od, n,
+                               mx.as<uint32_t>());
+    };
+    Captured orient_graph;
+    orient_graph.capture(s, enqueue);
+    if (scan_rc) return scan_rc;
+    KC_HIP(orient_graph.launch(s, enqueue));
+    if (scan_rc) return scan_rc;
+    KC_HIP(hipEventRecord(ev.e1, s));
+    uint32_t res[2] = {0, 0};  // m, max out-degree
+    KC_HIP(hipMemcpyAsync(&res[0], o->rowptr.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    KC_HIP(hipMemcpyAsync(&res[1], mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) return fail(KC_ERR_DEVICE, "orientation failed: %s", hipGetErrorString(se));
+    if (res[0] != m_expect)
+        return fail(KC_ERR_DEVICE, "orientation kept %u of %u edges (internal error)", res[0], m_expect);
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (se != hipSuccess) return bail(fail(KC_ERR_DEVICE, "orientation failed: %s", hipGetErrorString(se)));
-    o->max_out = maxout;
+    hipEventElapsedTime(&ms, ev.e0, ev.e1);
+    o->m = res[0];
+    o->max_out = res[1];
     o->max_core = max_core;
     o->pre_ms = ms;
-    *out = o;
+    *out = o.release();
     return KC_OK;
 }
 

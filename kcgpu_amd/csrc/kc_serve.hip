@@ -17,14 +17,25 @@
 //     device's resident block slots, so a long batch (a dense graph at a large
 //     k) leaves room for the short batches behind it instead of filling every
 //     CU until it ends.
-//   * Latency is measured from submit to the moment the worker has the result.
+//   * Pivoting queries on graphs whose tasks all fit 64 vertices (every
+//     query of most serving mixes) skip the workers: the engine thread
+//     appends their tasks to the open queues of one resident launch
+//     (kc::OpenEngine) while it runs, so a query that arrives while others
+//     are on the GPU is drained by the waves already resident, next to them.
+//     Queries of at most EXPRESS_TASKS tasks go to the queue the waves try
+//     first, so a short query is not held behind a long one.  A query is
+//     admitted when a query slot and queue room are free; each query's
+//     result comes back through its slot's mailbox in host memory.
+//   * Latency is measured from submit to the moment the result is on the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -38,6 +49,7 @@ namespace {
 
 struct Resident {
     kc_oriented *o = nullptr;
+    uint64_t epoch = 0;  // order of arrival among the server's graphs
     uint64_t bytes = 0;
     uint32_t pending = 0;
     bool removed = false;
@@ -61,6 +73,14 @@ struct Done {
 
 struct kc_server {
     kc_server_config cfg{};
+    std::unique_ptr<kc::OpenEngine> eng;  // resident launch with open queues (null: workers only)
+    std::deque<Ticket> open_pending;
+    std::thread engine;
+    uint64_t admissions = 0, admitted = 0, joined_running = 0, launches = 0;
+    // graphs added so far, and how many there were when the resident launch
+    // started: a graph uploaded after that start is counted by the workers
+    // until the launch restarts (a launch sees memory written before it began)
+    uint64_t graph_epoch = 0, launch_epoch = 0;
     int dev = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -80,6 +100,31 @@ struct kc_server {
 };
 
 namespace {
+
+constexpr uint64_t EXPRESS_TASKS = 1u << 16;
+
+uint64_t task_count(const kc_oriented *o, const kc_query &q) { return q.gran == KC_GRAN_EDGE ? o->m : o->n; }
+
+// can the resident launch take this query
+bool open_eligible(const kc_server *s, const kc_oriented *o, const kc_query &q)
+{
+    return s->eng && q.algo == KC_ALGO_PIVOT && q.k >= 3 && o->max_out <= kc::OPEN_MAX_ROWS;
+}
+
+// record a finished query (caller holds s->mu)
+void complete(kc_server *s, const Ticket &t, const Done &d)
+{
+    s->lat.push_back(d.latency_ms);
+    s->done[t.id] = d;
+    Resident &g = s->graphs[t.gid];
+    if (--g.pending == 0 && g.removed) {
+        s->resident_bytes -= g.bytes;
+        kc_oriented_free(g.o);
+        s->graphs.erase(t.gid);
+    }
+    s->completed++;
+    s->st_completed++;
+}
 
 uint64_t oriented_bytes(uint32_t n, uint64_t m) { return 4ull * (n + 1) + 8ull * m + 4ull * n + 4096; }
 
@@ -157,23 +202,154 @@ void worker_main(kc_server *s)
                     if (d.r.overflow) d.rc = KC_ERR_OVERFLOW;
                 }
                 d.latency_ms = std::chrono::duration<double, std::milli>(t_end - batch[i].t_submit).count();
-                s->lat.push_back(d.latency_ms);
-                s->done[batch[i].id] = d;
-                Resident &g = s->graphs[batch[i].gid];
-                if (--g.pending == 0 && g.removed) {
-                    s->resident_bytes -= g.bytes;
-                    kc_oriented_free(g.o);
-                    s->graphs.erase(batch[i].gid);
-                }
+                complete(s, batch[i], d);
             }
-            s->completed += batch.size();
-            s->st_completed += batch.size();
             s->t_last = t_end;
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
     hipStreamDestroy(st);
+}
+
+
+// The engine thread: admits queued queries into the resident launches' queues
+// (express: at most EXPRESS_TASKS tasks), starts a queue's launch while it has
+// queries in flight, ends it after STOP_IDLE without any, and collects results
+// from the mailboxes.
+void engine_main(kc_server *s)
+{
+    hipSetDevice(s->dev);
+    kc::OpenEngine &e = *s->eng;
+    // an idle launch ends after this long, so its waves free their CUs for the
+    // workers' launches (an express launch kept up for a second between
+    // queries slowed the heavy queries on the workers: serving mix at 8
+    // queries/s, p99 140 ms -> 11.7 s)
+    const std::chrono::microseconds STOP_IDLE[kc::OpenEngine::NQ] = {std::chrono::microseconds(2000),
+                                                                     std::chrono::microseconds(2000)};
+    struct Flight {
+        Ticket t;
+        uint32_t seq;
+        int queue;
+    };
+    std::unordered_map<uint32_t, Flight> inflight;  // by slot
+    Clock::time_point last_active[kc::OpenEngine::NQ] = {Clock::now(), Clock::now()};
+    uint64_t launch_epoch[kc::OpenEngine::NQ] = {0, 0};
+    for (;;) {
+        std::vector<Ticket> take[kc::OpenEngine::NQ];
+        std::vector<kc::OpenQueryIn> in[kc::OpenEngine::NQ];
+        bool any_running = false;
+        for (int j = 0; j < kc::OpenEngine::NQ; ++j) any_running |= e.running(j);
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            auto has_work = [&] { return s->stop || !s->open_pending.empty(); };
+            if (inflight.empty() && !any_running) s->cv_work.wait(lk, has_work);
+            else s->cv_work.wait_for(lk, std::chrono::microseconds(20), has_work);
+            if (s->stop && s->open_pending.empty() && inflight.empty()) break;
+            for (int j = 0; j < kc::OpenEngine::NQ; ++j)  // indices restart at 0 when a queue is idle
+                if (!e.qs[j].inflight && e.qs[j].nseg && !e.running(j)) e.reset(j);
+            size_t slots = e.free_slots.size();
+            while (!s->open_pending.empty() && slots > 0) {
+                const Ticket &t = s->open_pending.front();
+                const Resident &r = s->graphs[t.gid];
+                const int j = task_count(r.o, t.q) <= EXPRESS_TASKS ? 0 : 1;
+                // a launch sees memory written before it began: a graph uploaded
+                // since goes to the workers while that launch runs
+                if (e.running(j) && !e.qs[j].stopped && r.epoch > launch_epoch[j]) {
+                    s->pending.push_back(t);
+                    s->open_pending.pop_front();
+                    s->cv_work.notify_all();
+                    continue;
+                }
+                if (e.room(j) <= take[j].size()) break;  // waits for the queue to drain and restart
+                in[j].push_back(kc::OpenQueryIn{r.o, t.q, (uint32_t)(t.id % 0xFFFFFFFEull) + 1});
+                take[j].push_back(t);
+                s->open_pending.pop_front();
+                --slots;
+            }
+        }
+        for (int j = 0; j < kc::OpenEngine::NQ; ++j) {
+            if (take[j].empty()) continue;
+            // other queries on the GPU in a resident launch: this one joins them
+            bool was_running = false;
+            for (int i = 0; i < kc::OpenEngine::NQ; ++i) was_running |= e.running(i) && !e.qs[i].stopped;
+            std::vector<uint32_t> slot;
+            e.admit(j, in[j], slot);
+            const auto now = Clock::now();
+            last_active[j] = now;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->admissions++;
+            s->admitted += take[j].size();
+            if (was_running) s->joined_running += take[j].size();
+            for (size_t i = 0; i < take[j].size(); ++i) {
+                if (slot[i] == ~0u) {  // no task: the count is 0
+                    Done d;
+                    d.r.algo = take[j][i].q.algo;
+                    d.r.gran = take[j][i].q.gran;
+                    d.r.part_size = take[j][i].q.part_size;
+                    d.latency_ms = std::chrono::duration<double, std::milli>(now - take[j][i].t_submit).count();
+                    complete(s, take[j][i], d);
+                    s->t_last = now;
+                } else {
+                    inflight[slot[i]] = Flight{take[j][i], in[j][i].seq, j};
+                }
+            }
+            s->cv_done.notify_all();
+        }
+        // launches: start a queue's launch while it has work, end it when idle
+        for (int j = 0; j < kc::OpenEngine::NQ; ++j) {
+            const bool run = e.running(j);
+            if (e.qs[j].inflight && !run) {
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    launch_epoch[j] = s->graph_epoch;
+                }
+                if (e.launch(j) == KC_OK) {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->launches++;
+                }
+                last_active[j] = Clock::now();
+            } else if (run && !e.qs[j].stopped && !e.qs[j].inflight &&
+                       (s->stop || Clock::now() - last_active[j] > STOP_IDLE[j])) {
+                e.stop(j);
+            }
+        }
+        // results
+        std::vector<std::pair<Flight, kc::OpenMail>> fin;
+        for (auto it = inflight.begin(); it != inflight.end();) {
+            volatile kc::OpenMail *m = e.mail + it->first;
+            if (m->seq == it->second.seq) {
+                std::atomic_thread_fence(std::memory_order_acquire);
+                kc::OpenMail r{m->count, m->ovf, m->seq};
+                fin.emplace_back(it->second, r);
+                e.release(it->first);
+                e.qs[it->second.queue].inflight--;
+                last_active[it->second.queue] = Clock::now();
+                it = inflight.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (!fin.empty()) {
+            const auto now = Clock::now();
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto &f : fin) {
+                Done d;
+                d.r.count = f.second.count;
+                d.r.overflow = f.second.ovf ? 1 : 0;
+                d.r.algo = f.first.t.q.algo;
+                d.r.gran = f.first.t.q.gran;
+                d.r.part_size = f.first.t.q.part_size;
+                d.rc = d.r.overflow ? KC_ERR_OVERFLOW : KC_OK;
+                d.latency_ms = std::chrono::duration<double, std::milli>(now - f.first.t.t_submit).count();
+                complete(s, f.first.t, d);
+            }
+            s->t_last = now;
+            s->cv_done.notify_all();
+        }
+    }
+    for (int j = 0; j < kc::OpenEngine::NQ; ++j)
+        if (e.running(j) && !e.qs[j].stopped) e.stop(j);
 }
 
 }  // namespace
@@ -200,6 +376,18 @@ int kc_server_create(const kc_server_config *cfg, kc_server **out)
         s->cfg.budget_bytes = (uint64_t)(fr * 0.9);
     }
     s->dev = ctx->dev;
+    // the resident launch's slot sums and binomial table come out of the budget
+    const uint64_t eng_bytes = 1ull << 20;
+    if (s->cfg.budget_bytes > (64ull << 20)) {
+        auto e = std::make_unique<OpenEngine>();
+        if ((rc = e->init(s->dev))) {
+            delete s;
+            return rc;
+        }
+        s->eng = std::move(e);
+        s->resident_bytes += eng_bytes;
+        s->engine = std::thread(engine_main, s);
+    }
     for (uint32_t i = 0; i < s->cfg.workers; ++i) s->workers.emplace_back(worker_main, s);
     *out = s;
     return KC_OK;
@@ -230,6 +418,7 @@ int kc_server_add_graph(kc_server *s, const kc_graph *g, int orient_mode, uint32
     Resident r;
     r.o = o;
     r.bytes = foot;
+    r.epoch = ++s->graph_epoch;
     *gid = s->next_gid++;
     s->graphs[*gid] = r;
     return KC_OK;
@@ -287,10 +476,11 @@ int kc_server_submit(kc_server *s, uint32_t gid, const kc_query *q, uint64_t *ti
         s->t_first = t.t_submit;
         s->any = true;
     }
-    s->pending.push_back(t);
+    if (open_eligible(s, it->second.o, t.q)) s->open_pending.push_back(t);
+    else s->pending.push_back(t);
     s->submitted++;
     *ticket = t.id;
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return KC_OK;
 }
 
@@ -328,7 +518,9 @@ int kc_server_drain(kc_server *s)
 {
     if (!s) return fail(KC_ERR_ARG, "kc_server_drain: null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->completed == s->submitted; });
+    s->cv_done.wait(lk, [&] {
+        return s->pending.empty() && s->open_pending.empty() && s->completed == s->submitted;
+    });
     return KC_OK;
 }
 
@@ -339,8 +531,11 @@ int kc_server_get_stats(kc_server *s, kc_server_stats *st)
     std::memset(st, 0, sizeof *st);
     st->submitted = s->submitted;
     st->completed = s->st_completed;
-    st->batches = s->batches;
-    st->mean_batch = s->batches ? (double)s->batched_queries / s->batches : 0.0;
+    st->batches = s->batches + s->admissions;
+    st->mean_batch = st->batches ? (double)(s->batched_queries + s->admitted) / st->batches : 0.0;
+    st->resident_admitted = s->admitted;
+    st->joined_running = s->joined_running;
+    st->resident_launches = s->launches;
     st->graphs = s->graphs.size();
     st->resident_bytes = s->resident_bytes;
     std::vector<double> l = s->lat;
@@ -369,6 +564,7 @@ int kc_server_reset_stats(kc_server *s)
     s->st_completed = 0;
     s->batches = 0;
     s->batched_queries = 0;
+    s->admissions = s->admitted = s->joined_running = s->launches = 0;
     s->any = false;
     return KC_OK;
 }
@@ -382,6 +578,8 @@ void kc_server_destroy(kc_server *s)
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
+    if (s->engine.joinable()) s->engine.join();
+    s->eng.reset();  // waits for the resident launch to end
     for (auto &kv : s->graphs) kc_oriented_free(kv.second.o);
     delete s;
 }

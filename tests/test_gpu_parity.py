@@ -229,15 +229,20 @@ def test_near_clique_closed_forms_k5_to_11(near_cliques, mode, side_by_side, mon
 
 
 def test_measured_autotune_plan_counts():
-    # a graph with out-degrees above 64: the plan is measured; counting with
-    # it gives the oracle's count
+    # a graph with out-degrees above 64: the plan is measured (orientation,
+    # granularity, and at k <= 5 the algorithm and partition size as well);
+    # counting with it gives the oracle's count
     G = K.Graph.generate(K.GEN_CHUNG_LU, seed=1, n=100000, m=600000, gamma=2.1, max_degree=5000.0)
     rp, col = _host_csr(G)
     orp, oc = O.orient(rp, col)
-    for k in (5, 6):
+    for k in (4, 5, 6):
         plan = G.autotune(k)
-        assert plan["algo"] == "pivot" and plan["orient"] in ("degree", "degen") and plan["gran"] in GRANS
-        r = G.orient(plan["orient"]).count(k, "pivot", plan["gran"], part_size=plan["part_size"])
+        print(k, plan, flush=True)
+        assert plan["orient"] in ("degree", "degen") and plan["gran"] in GRANS
+        assert plan["part_size"] in (1, 8, 64) and plan["algo"] in ALGOS
+        if k > 5:
+            assert plan["algo"] == "pivot" and plan["part_size"] == 1
+        r = G.orient(plan["orient"]).count(k, plan["algo"], plan["gran"], part_size=plan["part_size"])
         assert r.count == O.count(orp, oc, k, "pivot_tails", "edge")["count"], (k, plan)
 
 

@@ -1,0 +1,114 @@
+"""Serving through the resident launch (continuous batching), against kc_count
+alone and the oracle, through the C ABI."""
+import time
+
+import numpy as np
+import pytest
+
+import kcgpu_amd as K
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RMAT = dict(edge_factor=16, a=0.57, b=0.19, c=0.19)
+
+
+def _wait_in_order(srv, tickets, limit_s=120.0):
+    """poll until every ticket is done; (completion order, results by ticket)"""
+    order, res = [], {}
+    pending = set(tickets)
+    t0 = time.perf_counter()
+    while pending and time.perf_counter() - t0 < limit_s:
+        for t in list(pending):
+            r = srv.poll(t)
+            if r is not None:
+                order.append(t)
+                res[t] = r
+                pending.discard(t)
+        time.sleep(0.0005)
+    assert not pending, f"{len(pending)} queries still pending after {limit_s} s"
+    return order, res
+
+
+def test_short_queries_join_running_work_and_finish_first():
+    """A long query (R-MAT scale 16, degeneracy order, edge tasks at k=9;
+    tasks above 64 vertices, so a worker counts it with the size-class
+    kernels) is on the GPU when 4 medium queries (Barabasi-Albert n=100,000,
+    100,000 node tasks: the bulk queue of the resident launch) and 24 short
+    ones (n=3,000: the express queue) arrive.  The short and medium queries
+    join the running resident launches and finish before the long query; every
+    count equals kc_count on the same oriented graph alone, and the short
+    ones equal the oracle."""
+    longG = K.Graph.generate(K.GEN_RMAT, seed=3, scale=16, **RMAT)
+    med = K.Graph.generate(K.GEN_BA, seed=99, n=100000, attach=20)
+    shorts = [K.Graph.generate(K.GEN_BA, seed=100 + i, n=3000, attach=6 + i % 5) for i in range(6)]
+    srv = K.Server(workers=2)
+    try:
+        gl = srv.add_graph(longG, "degen")
+        gm = srv.add_graph(med, "degen")
+        gs = [srv.add_graph(G, "degen") for G in shorts]
+        tl = srv.submit(gl, 9, "pivot", "edge")
+        time.sleep(0.05)  # the long query is on the GPU
+        mq = [(gm, k, "pivot", "node") for k in (5, 6, 7, 8)]
+        tm = [srv.submit(*q) for q in mq]
+        time.sleep(0.01)  # the medium queries are in the resident launch
+        sq = [(gs[i % 6], 4 + i % 4, "pivot", ("node", "edge")[i % 2]) for i in range(24)]
+        ts = [srv.submit(*q) for q in sq]
+        order, res = _wait_in_order(srv, [tl] + tm + ts)
+        st = srv.stats()
+        print("order of the long query:", order.index(tl), "of", len(order), st, flush=True)
+        assert order[-1] == tl
+        assert st["resident_admitted"] == 28 and st["joined_running"] >= 24
+        Ol = longG.orient("degen")
+        assert res[tl].count == Ol.count(9, "pivot", "edge").count
+        Om = med.orient("degen")
+        for t, (_, k, a, gr) in zip(tm, mq):
+            assert res[t].count == Om.count(k, a, gr).count, k
+        ors = [G.orient("degen") for G in shorts]
+        for t, (g, k, a, gr) in zip(ts, sq):
+            i = gs.index(g)
+            assert res[t].count == ors[i].count(k, a, gr).count, (i, k, gr)
+        for i in range(3):
+            rp, col, _ = ors[i].export()
+            for k in (4, 5, 6, 7):
+                exp = O.count(rp.astype(np.uint64), col, k, "pivot_tails", "node")["count"]
+                got = [res[t].count for t, q in zip(ts, sq) if q[0] == gs[i] and q[1] == k]
+                assert all(x == exp for x in got), (i, k, got, exp)
+    finally:
+        srv.close()
+
+
+def test_graph_uploaded_mid_launch_and_mixed_modes():
+    """Queries on a graph added while the resident launch runs (counted by the
+    workers until the launch restarts), orientation-mode queries (workers)
+    and pivoting queries (resident launch) interleaved: every count equals
+    kc_count alone; then, once the server is idle, the new graph's queries go
+    to the resident launch too."""
+    base = [K.Graph.generate(K.GEN_CHUNG_LU, seed=20 + i, n=40000, m=200000, gamma=2.1, max_degree=800.0)
+            for i in range(3)]
+    late = K.Graph.generate(K.GEN_ER, seed=31, n=20000, m=150000)
+    srv = K.Server(workers=2)
+    try:
+        gids = [srv.add_graph(G, "degen") for G in base]
+        ors = [G.orient("degen") for G in base]
+        qs = [(j % 3, 4 + j % 5, ("pivot", "orient")[j % 3 == 2], ("node", "edge")[j % 2]) for j in range(30)]
+        tickets = [srv.submit(gids[i], k, a, gr) for i, k, a, gr in qs]
+        glate = srv.add_graph(late, "degen")
+        olate = late.orient("degen")
+        t_late = [srv.submit(glate, k, "pivot", "edge") for k in (3, 4, 5)]
+        order, res = _wait_in_order(srv, tickets + t_late)
+        for t, (i, k, a, gr) in zip(tickets, qs):
+            assert res[t].count == ors[i].count(k, a, gr, part_size=1).count, (i, k, a, gr)
+        for t, k in zip(t_late, (3, 4, 5)):
+            assert res[t].count == olate.count(k, "pivot", "edge").count
+        srv.drain()
+        time.sleep(0.01)
+        before = srv.stats()["resident_admitted"]
+        t2 = [srv.submit(glate, k, "pivot", "node") for k in (3, 4, 5)]
+        _, res2 = _wait_in_order(srv, t2)
+        assert srv.stats()["resident_admitted"] == before + 3
+        for t, k in zip(t2, (3, 4, 5)):
+            assert res2[t].count == olate.count(k, "pivot", "node").count
+        print(srv.stats(), flush=True)
+    finally:
+        srv.close()
