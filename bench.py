@@ -514,7 +514,10 @@ def main():
         c = last[k].count
         per_k[str(k)] = {"count": c, "device_ms": round(ms, 3), "kcliques_per_s": c / (ms / 1e3) if ms else None,
                          "edges_per_s": m_or / (ms / 1e3) if ms else None, "tasks": last[k].raw["tasks"],
-                         "tree_nodes": last[k].raw["tree_nodes"], "part_size": last[k].raw["part_size"]}
+                         "tree_nodes": last[k].raw["tree_nodes"], "part_size": last[k].raw["part_size"],
+                         # the pivoting kernels run one task per 64-lane wave; the
+                         # partition size shapes only the orientation DFS
+                         "lanes_per_task": 64}
 
     # dominant kernel symbol: the size class whose tasks hold the device's
     # waves longest over the step (summed per-task wave time; the launches of
@@ -537,19 +540,31 @@ def main():
     avg_ms = cls_ms[dc] / max(cls_launch[dc], 1)
     avg_bytes = cls_bytes[dc] / max(cls_launch[dc], 1)
     achieved = avg_bytes / (avg_ms / 1e3) / 1e9 if avg_ms else 0.0
-    traffic = None
+    # HBM traffic and issue rates of the dominant kernel come from rocprofv3
+    # counter runs of this same command (profiles/): counters cannot be read
+    # inside a timed run
+    traffic, issue = None, None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        issue = {"valu_issue_frac": tj.get("valu_issue_frac"),
+                 "valu_insts_per_simd_cycle": tj.get("valu_insts_per_simd_cycle"),
+                 "valu_issue_ceiling": 0.25,
+                 "salu_insts_per_cu_cycle": tj.get("salu_insts_per_cu_cycle"),
+                 "lds_bank_conflict_frac": tj.get("lds_bank_conflict_frac"),
+                 "lds_conflict_cycles_per_lds_inst": tj.get("lds_conflict_cycles_per_lds_inst"),
+                 "kernel": tj.get("kernel"), "source": tj.get("source")}
     names = {1: "k_count<pivot,LDS>", 2: f"k_pivot_reg<{[1, 2, 4][dc] if dc < 3 else '?'}>", 3: "k_count<pivot,HBM>",
              4: "k_pivot_wide<LDS>", 5: "k_pivot_wide<HBM>"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": f"{names.get(cls_var[dc], '?')} (size class {dc}, tasks of <= {[64, 128, 256, 512, 1024, 'more'][dc]} vertices)",
                 "launches_per_step": int(cls_launch[dc]), "avg_launch_ms": round(avg_ms, 3),
-                "alg_bytes_per_launch": int(avg_bytes),
-                "note": "instruction-issue-bound (VALU + SALU) SCT on a dense core: algorithmic HBM bytes are tiny next to the bitmap work"}
+                "alg_bytes_per_launch": int(avg_bytes), "issue": issue,
+                "note": "instruction-issue-bound (VALU + SALU) SCT on a dense core: algorithmic HBM bytes are tiny "
+                        "next to the bitmap work; issue.valu_issue_frac is the bound that governs this kernel"}
 
     cpu = None
     if not args.no_cpu:
