@@ -168,6 +168,25 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out);
  * process, src/main.cu:73-235).
  */
 int kc_orient_batch(const kc_graph *const *gs, const int *modes, size_t nb, kc_oriented **out);
+
+/*
+ * Build the simple undirected CSR of an edge list on the device (the same
+ * graph as kc_graph_from_edges: self-loops dropped, both directions, rows
+ * sorted, duplicates removed; n = 0: max id + 1) and orient it there, without
+ * a host CSR: the edge list is uploaded once, then degree histogram, scan,
+ * scatter, per-row sort and deduplication (a wave's bitonic sort in LDS, or a
+ * bitmap over the vertex range for rows above 2048 entries), compaction and
+ * orientation all run on the GPU.  Replaces CSRCOO::from_edgelist
+ * (include/CSRCOO.cuh:37-80), the optional edge sort
+ * (include/FIleReader.cuh:314-320) and src/main.cu:84-235.
+ */
+int kc_orient_edges(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst, int mode,
+                    kc_oriented **out);
+
+/* kc_graph_from_edges with the CSR built on the device (as in kc_orient_edges)
+ * and copied back into a host graph. */
+int kc_graph_from_edges_device(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst,
+                               kc_graph **out);
 int kc_oriented_get_info(const kc_oriented *o, kc_oriented_info *info);
 /* Copy the oriented CSR (rowptr[n+1], col[m]) and the orientation key
  * (degree or peel round, key[n]) to the host; any pointer may be NULL. */

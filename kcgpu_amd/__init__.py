@@ -132,6 +132,10 @@ SIGNATURES = {
     "kc_orient": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "kc_orient_batch": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
                                        ctypes.POINTER(_vp)]),
+    "kc_orient_edges": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_size_t, _u32p, _u32p, ctypes.c_int,
+                                       ctypes.POINTER(_vp)]),
+    "kc_graph_from_edges_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_size_t, _u32p, _u32p,
+                                                  ctypes.POINTER(_vp)]),
     "kc_oriented_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(OrientedInfo)]),
     "kc_oriented_export": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
     "kc_oriented_free": (None, [_vp]),
@@ -245,6 +249,17 @@ class Graph:
         return cls(h.value)
 
     @classmethod
+    def from_edges_device(cls, src, dst, n=0):
+        """from_edges with the CSR built on the GPU (kc_graph_from_edges_device)."""
+        s, sp = _u32(src)
+        d, dp = _u32(dst)
+        if s.size != d.size:
+            raise ValueError("src and dst differ in length")
+        h = ctypes.c_void_p()
+        _check(lib().kc_graph_from_edges_device(int(n), s.size, sp, dp, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def from_csr(cls, rowptr, col):
         r, rp = _u32(rowptr)
         c, cp = _u32(col)
@@ -283,6 +298,18 @@ class Graph:
         h = ctypes.c_void_p()
         _check(lib().kc_orient(self._h, ORIENT[mode] if isinstance(mode, str) else int(mode), ctypes.byref(h)))
         return Oriented(h.value)
+
+
+def orient_edges(src, dst, mode="degree", n=0):
+    """Oriented graph of an edge list, with the CSR built and oriented on the
+    GPU (kc_orient_edges): no host CSR."""
+    s, sp = _u32(src)
+    d, dp = _u32(dst)
+    if s.size != d.size:
+        raise ValueError("src and dst differ in length")
+    h = ctypes.c_void_p()
+    _check(lib().kc_orient_edges(int(n), s.size, sp, dp, ORIENT[mode], ctypes.byref(h)))
+    return Oriented(h.value)
 
 
 def orient_batch(graphs, modes):

@@ -107,6 +107,16 @@ struct LaunchStats {
 int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, std::vector<kc_result> &res,
                 LaunchStats *ls, uint64_t *local = nullptr, uint32_t grid_share = 1);
 
+// The simple undirected CSR of a raw edge list built on the device
+// (kc_build.hip): rowptr[n+1] and col[nnz] in HBM, the same graph as the host
+// builder.  n_in = 0 derives max id + 1.
+int build_csr_device(DeviceCtx *ctx, hipStream_t s, uint32_t n_in, size_t ne, const uint32_t *src,
+                     const uint32_t *dst, DevBuf &rowptr, DevBuf &col, uint32_t &n, uint64_t &nnz, uint32_t &maxdeg);
+
+// Orient a simple undirected CSR already in HBM (kc_prep.hip).
+int orient_device_csr(DeviceCtx *ctx, uint32_t n, uint32_t nnz, uint32_t maxdeg, const uint32_t *frow,
+                      const uint32_t *fcol, int mode, kc_oriented **out);
+
 // query validation and the autotuner's choice of algo / gran / P for a query
 int check_query(const kc_query *q);
 kc_query tune_query(const kc_oriented *o, const kc_query *q);

@@ -851,6 +851,27 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     if (rc) return rc;
     const uint32_t n = g->csr.n;
     const uint32_t nnz = (uint32_t)g->csr.col.size();
+    hipStream_t s = ctx->stream;
+    DevBuf frow, fcol;
+    if ((rc = frow.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = fcol.ensure(sizeof(uint32_t) * (size_t)nnz + 4)))
+        return rc;
+    if (hipMemcpyAsync(frow.p, g->csr.rowptr.data(), sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        (nnz && hipMemcpyAsync(fcol.p, g->csr.col.data(), sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice, s) !=
+                    hipSuccess))
+        return fail(KC_ERR_DEVICE, "graph upload failed");
+    return orient_device_csr(ctx, n, nnz, g->max_degree, frow.as<uint32_t>(), fcol.as<uint32_t>(), mode, out);
+}
+
+}  // extern "C"
+
+// the orientation of a simple undirected CSR already in HBM (frow, fcol are
+// only read; they may be freed once this returns)
+int kc::orient_device_csr(DeviceCtx *ctx, uint32_t n, uint32_t nnz, uint32_t maxdeg, const uint32_t *frow_p,
+                          const uint32_t *fcol_p, int mode, kc_oriented **out)
+{
+    *out = nullptr;
+    int rc;
     // the CSR is simple and symmetric, so every undirected edge is kept once
     const uint32_t m_expect = nnz / 2;
     std::unique_ptr<kc_oriented> o(new kc_oriented);
@@ -862,19 +883,15 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     // every buffer is sized up front, so the launches need no host read in
     // between and run as captured graphs
     DevBuf frow, fcol, outdeg, tmp, mx;
+    frow.view(const_cast<uint32_t *>(frow_p), sizeof(uint32_t) * ((size_t)n + 1));
+    fcol.view(const_cast<uint32_t *>(fcol_p), sizeof(uint32_t) * (size_t)nnz + 4);
     const size_t np = ((size_t)n + 2047) / 2048;
-    if ((rc = frow.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = fcol.ensure(sizeof(uint32_t) * (size_t)nnz + 4)) ||
-        (rc = outdeg.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = o->key.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
+    if ((rc = outdeg.ensure(sizeof(uint32_t) * ((size_t)n + 1))) || (rc = o->key.ensure(sizeof(uint32_t) * (size_t)n + 4)) ||
         (rc = o->rowptr.ensure(sizeof(uint32_t) * ((size_t)n + 1))) ||
         (rc = o->col.ensure(sizeof(uint32_t) * (size_t)m_expect + 4)) ||
         (rc = o->src.ensure(sizeof(uint32_t) * (size_t)m_expect + 4)) || (rc = tmp.ensure(sizeof(uint64_t) * (np + 1))) ||
         (rc = mx.ensure(sizeof(uint32_t))))
         return rc;
-    if (hipMemcpyAsync(frow.p, g->csr.rowptr.data(), sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) !=
-            hipSuccess ||
-        (nnz && hipMemcpyAsync(fcol.p, g->csr.col.data(), sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice, s) !=
-                    hipSuccess))
-        return fail(KC_ERR_DEVICE, "graph upload failed");
     struct Events {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ~Events()
@@ -888,7 +905,7 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     KC_HIP(hipEventRecord(ev.e0, s));
     uint32_t max_core = 0;
     if (mode == KC_ORIENT_DEGEN && n > 0 &&
-        (rc = run_peel(ctx, s, n, nnz, g->max_degree, frow.as<uint32_t>(), fcol.as<uint32_t>(), o->key.as<uint32_t>(),
+        (rc = run_peel(ctx, s, n, nnz, maxdeg, frow.as<uint32_t>(), fcol.as<uint32_t>(), o->key.as<uint32_t>(),
                        &max_core)))
         return rc;
     // keep flags, per-row counts, scan, stable compaction, max out-degree:
@@ -939,6 +956,51 @@ int kc_orient(const kc_graph *g, int mode, kc_oriented **out)
     o->max_core = max_core;
     o->pre_ms = ms;
     *out = o.release();
+    return KC_OK;
+}
+
+extern "C" {
+
+// Build the CSR of a raw edge list on the device and orient it there: the
+// host never holds the CSR.
+int kc_orient_edges(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst, int mode, kc_oriented **out)
+{
+    if (!out || (nedges && (!src || !dst))) return fail(KC_ERR_ARG, "kc_orient_edges: null argument");
+    if (mode != KC_ORIENT_DEGREE && mode != KC_ORIENT_DEGEN) return fail(KC_ERR_ARG, "kc_orient_edges: bad mode %d", mode);
+    *out = nullptr;
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    DevBuf rp, col;
+    uint32_t nv = 0, maxdeg = 0;
+    uint64_t nnz = 0;
+    if ((rc = build_csr_device(ctx, ctx->stream, n, nedges, src, dst, rp, col, nv, nnz, maxdeg))) return rc;
+    rc = orient_device_csr(ctx, nv, (uint32_t)nnz, maxdeg, rp.as<uint32_t>(), col.as<uint32_t>(), mode, out);
+    hipStreamSynchronize(ctx->stream);  // rp and col are freed on return
+    return rc;
+}
+
+// kc_graph_from_edges with the CSR built on the device and copied back
+int kc_graph_from_edges_device(uint32_t n, size_t nedges, const uint32_t *src, const uint32_t *dst, kc_graph **out)
+{
+    if (!out || (nedges && (!src || !dst))) return fail(KC_ERR_ARG, "kc_graph_from_edges_device: null argument");
+    *out = nullptr;
+    DeviceCtx *ctx;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    DevBuf rp, col;
+    uint32_t nv = 0, maxdeg = 0;
+    uint64_t nnz = 0;
+    if ((rc = build_csr_device(ctx, ctx->stream, n, nedges, src, dst, rp, col, nv, nnz, maxdeg))) return rc;
+    std::unique_ptr<kc_graph> g(new kc_graph);
+    g->csr.n = nv;
+    g->csr.rowptr.resize((size_t)nv + 1);
+    g->csr.col.resize(nnz);
+    KC_HIP(hipMemcpyAsync(g->csr.rowptr.data(), rp.p, 4 * ((size_t)nv + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (nnz) KC_HIP(hipMemcpyAsync(g->csr.col.data(), col.p, 4 * nnz, hipMemcpyDeviceToHost, ctx->stream));
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    g->max_degree = maxdeg;
+    *out = g.release();
     return KC_OK;
 }
 
