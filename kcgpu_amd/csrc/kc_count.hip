@@ -767,7 +767,6 @@ __device__ uint64_t pivot_count(const CountArgs &a, uint32_t P, uint32_t k, cons
         }
     };
     for (;;) {
-        if (uni32(ovf)) break;  // the count is past 2^64: the rest of the subtree cannot matter
         if (pending) {
             pending = false;
             ++my_nodes;
@@ -1537,28 +1536,6 @@ __device__ __attribute__((always_inline)) void finish_wave(const CountArgs &a, W
     }
 }
 
-// A query whose count has reached 2^64 is dead: its result is the overflow
-// flag, so the rest of its tasks is skipped.  Each lane loads the same word,
-// so the answer is the wave's.
-__device__ __attribute__((always_inline)) bool query_dead(const CountArgs &a, uint32_t q)
-{
-    return uni32(__hip_atomic_load(&a.qovf[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
-}
-
-// Tell the other waves as soon as this wave's share of its query overflows,
-// and add a running sum of 2^48 or more to the query's total at once, so that
-// the total wraps (and the query is dead) as soon as the counted part of it
-// reaches 2^64, not when the last wave flushes.
-__device__ __attribute__((always_inline)) void publish_dead(const CountArgs &a, WaveAcc &w)
-{
-    if (w.q == INF32) return;
-    if (w.run_ovf) {
-        if (lane_id() == 0) atomicOr(&a.qovf[w.q], 1u);
-    } else if (uni64(w.run) >> 48) {
-        flush_query(a, w);
-    }
-}
-
 // register-bitmap pivoting kernel for tasks of at most 64*S vertices
 // waves per SIMD the register budget is held to: 4 for S = 4 (<= 128 VGPRs), 6 for S = 2, 8 for S = 1
 template <int S>
@@ -1587,7 +1564,6 @@ void k_pivot_reg(CountArgs a)
             uint32_t task;
             const Cur g = select_task(a, a.tasks[ti], w, task);
             ++w.ntask;
-            if (query_dead(a, w.q)) continue;
             const uint32_t R = stage_task(g, task, list, w.bytes);
             if (g.gran && g.k == 3) {  // an edge task at k=3 adds |T| (no bitmap needed)
                 add_checked(w.run, R, w.run_ovf);
@@ -1598,7 +1574,6 @@ void k_pivot_reg(CountArgs a)
             build_rows<true>(g, list, offs, starts, rows, R, (R + 63) / 64, w.bytes);
             add_checked(w.run, pivot_reg<S>(a, g.k, rows, R, g.gran ? 2u : 1u, offs, w.nodes, w.run_ovf, 0u, crows),
                         w.run_ovf);
-            publish_dead(a, w);
             __syncthreads();
             const uint64_t dt = wall_clock64() - t0;
             w.tmax = dt > w.tmax ? dt : w.tmax;
@@ -1853,7 +1828,6 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
             const Cur g = select_task(a, a.tasks[ti], w, task);
             const uint32_t k = g.k;
             ++w.ntask;
-            if (query_dead(a, w.q)) continue;
             const uint32_t R = stage_task(g, task, list, w.bytes);
             if (LOCAL)
                 for (uint32_t i = lane; i < R; i += 64) lacc[i] = 0;
@@ -1887,7 +1861,6 @@ __global__ __launch_bounds__(64) void k_count(CountArgs a)
                                                 w.nodes, w.run_ovf, lacc);
                 add_checked(w.run, task_count, w.run_ovf);
             }
-            publish_dead(a, w);
             __syncthreads();
             publish_local<LOCAL>(a, g, task, list, lacc, R, task_count);
             if (LOCAL) __syncthreads();
@@ -1985,7 +1958,7 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, NW = blockDim.x >> 6;
     uint64_t *ws = GLOBAL ? a.gscr + (uint64_t)blockIdx.x * a.gscr_words : lds;
     const WideLayout L = make_wide_layout(a.rmax, a.wmax, a.levels, NW, a.chain);
-    uint32_t *ctl = reinterpret_cast<uint32_t *>(ws + L.ctl);  // task, R, next branch, branches, dead
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(ws + L.ctl);  // task, R, next branch, branches, pivot
     uint32_t *list = reinterpret_cast<uint32_t *>(ws + L.list);
     uint32_t *offs = reinterpret_cast<uint32_t *>(ws + L.offs);  // row offsets, then the entries (j << 16 | b)
     uint32_t *starts = reinterpret_cast<uint32_t *>(ws + L.starts);
@@ -2008,20 +1981,11 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
         const Cur g = select_task(a, a.tasks[ti], w, task);
         const uint32_t k = g.k, h0 = g.gran ? 2u : 1u;
         if (wv == 0) {
-            // a dead query's task is skipped by the whole block: wave 0 decides
-            const bool dead = query_dead(a, w.q);
-            const uint32_t r = dead ? 0u : stage_task(g, task, list, w.bytes);
-            if (lane == 0) {
-                ctl[1] = r;
-                ctl[4] = dead;
-            }
+            const uint32_t r = stage_task(g, task, list, w.bytes);
+            if (lane == 0) ctl[1] = r;
             ++w.ntask;
         }
         __syncthreads();
-        if (ctl[4]) {
-            __syncthreads();  // ctl is rewritten by the next task
-            continue;
-        }
         const uint32_t R = ctl[1], W = (R + 63) / 64;
         if (g.gran && k == 3) {  // an edge task at k=3 adds |T| (no bitmap needed)
             if (wv == 0) add_checked(w.run, R, w.run_ovf);
@@ -2104,7 +2068,6 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
                 ctl[2] = 0;
                 ctl[3] = ne;
             }
-            publish_dead(a, w);
         }
         __syncthreads();
         const uint32_t ne = ctl[3];
@@ -2113,7 +2076,6 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
             if (lane == 0) i = atomicAdd(&ctl[2], 1u);
             i = uni32(i);
             if (i >= ne) break;
-            if (query_dead(a, w.q)) continue;  // (the remaining entries are drained unrun)
             const uint32_t e = offs[i], j = e >> 16, b = e & 0xFFFFu;
             const uint64_t *Xj = chX + (uint64_t)j * 2 * a.wmax, *Bj = chB + (uint64_t)j * 2 * a.wmax;
             const bool end = (b == 0xFFFFu);
@@ -2132,7 +2094,6 @@ __global__ __launch_bounds__(512) void k_pivot_wide(CountArgs a)
                         pivot_count<false, true>(a, g.psize, k, rows, stk, X, scal, cand, R, W, h, w.nodes, w.run_ovf,
                                                  nullptr, j, false, crows2, crows4),
                         w.run_ovf);
-            publish_dead(a, w);
         }
         __syncthreads();
         if (wv == 0) {
