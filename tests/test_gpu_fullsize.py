@@ -153,14 +153,15 @@ def test_rmat_scale22_sampled_vs_oracle():
     """R-MAT scale 22, edge factor 16, (a, b, c, d) = (.57, .19, .19, .05),
     seed 3, deduplicated, symmetrized, loops dropped (64.2 M edges, degeneracy
     961): both orientations equal the oracle's entry for entry; strided task
-    samples at k = 3..7, node and edge tasks, equal the oracle's counts over
-    the same tasks; the full counts at k = 3..5 agree across the four
-    (orientation, granularity) modes."""
+    samples at k = 3..9 (node and edge tasks to k = 7, edge tasks at k = 8, 9)
+    equal the oracle's counts over the same tasks; the full counts at k = 3..5
+    agree across the four (orientation, granularity) modes."""
     t0 = time.perf_counter()
     G = K.Graph.generate(K.GEN_RMAT, seed=3, scale=22, **RMAT)
     _log("rmat s22", G.info(), f"{time.perf_counter() - t0:.1f}s")
     cases = {"degen": ((3, "edge", 97, 0), (4, "edge", 97, 1), (5, "node", 97, 2), (6, "node", 499, 3),
-                       (6, "edge", 4999, 4), (7, "node", 2003, 5), (7, "edge", 49999, 6)),
+                       (6, "edge", 4999, 4), (7, "node", 2003, 5), (7, "edge", 49999, 6),
+                       (8, "edge", 199999, 7), (9, "edge", 999983, 9)),
              "degree": ((4, "node", 97, 7), (5, "edge", 997, 8), (6, "node", 997, 9))}
     full = {}
     for mode, cs in cases.items():
