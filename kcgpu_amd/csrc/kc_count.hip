@@ -49,6 +49,7 @@ constexpr int NCLASS = 6;
 constexpr uint32_t CLASS_ROWS[NCLASS] = {64, 128, 256, 512, 1024, 0xFFFFFFFFu};
 constexpr uint32_t INF32 = 0xFFFFFFFFu;
 constexpr int ACC = 8;  // per-class accumulator slots
+constexpr uint32_t PIVOT_TOP2 = 256;  // largest pivoting task of the four-word register kernel
 
 // one query of a launch: the oriented graph it runs on and its k
 struct QDesc {
@@ -2145,9 +2146,11 @@ __global__ __launch_bounds__(256) void k_task_size(const QDesc *qd, const uint64
     }
 }
 
-__device__ inline int class_of(uint32_t s)
+// size class of a task; for pivoting, tasks above top2 (<= 256) go to the
+// block-wide kernel instead of the four-word register kernel
+__device__ inline int class_of(uint32_t s, uint32_t top2)
 {
-    return s <= 64 ? 0 : s <= 128 ? 1 : s <= 256 ? 2 : s <= 512 ? 3 : s <= 1024 ? 4 : 5;
+    return s <= 64 ? 0 : s <= 128 ? 1 : s <= top2 ? 2 : s <= 512 ? 3 : s <= 1024 ? 4 : 5;
 }
 
 // Bin tasks by (algorithm, size class, size slice); a task smaller than the
@@ -2161,10 +2164,10 @@ __device__ inline int class_of(uint32_t s)
 // one at a time (ballot per slice, one atomic per slice).
 constexpr int NSUB = 16;
 
-__device__ inline int slice_of(uint32_t s, int c)
+__device__ inline int slice_of(uint32_t s, int c, uint32_t top2)
 {
     if (c == 0 || c == NCLASS - 1) return 0;  // (the many small tasks: one slice keeps classification cheap)
-    const uint32_t lo = c == 0 ? 0u : CLASS_ROWS[c - 1], hi = CLASS_ROWS[c];
+    const uint32_t lo = c == 3 ? top2 : CLASS_ROWS[c - 1], hi = c == 2 ? top2 : CLASS_ROWS[c];
     const uint32_t sl = (hi - s) * NSUB / (hi - lo);
     return sl < NSUB ? (int)sl : NSUB - 1;
 }
@@ -2172,7 +2175,7 @@ __device__ inline int slice_of(uint32_t s, int c)
 __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t *qalgo, const uint64_t *qoff,
                                                    uint32_t nq, uint64_t total, const uint32_t *size, uint32_t *sub_cnt,
                                                    uint32_t *bin_max, uint32_t *sub_pos, const uint32_t *sub_off,
-                                                   uint64_t *out, int scatter)
+                                                   uint64_t *out, int scatter, uint32_t pivot_top2)
 {
     const int lane = threadIdx.x & 63;
     for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; base < total;
@@ -2186,8 +2189,9 @@ __global__ __launch_bounds__(256) void k_classify(const QDesc *qd, const uint8_t
             const uint32_t need = d.gran ? d.k - 2 : d.k - 1;
             s = size[gi];
             if (s >= need && (d.sample <= 1 || (uint32_t)(gi - qoff[q]) % d.sample == d.offset)) {
-                const int c = class_of(s);
-                fb = ((qalgo[q] ? NCLASS : 0) + c) * NSUB + slice_of(s, c);
+                const uint32_t top2 = qalgo[q] ? pivot_top2 : CLASS_ROWS[2];
+                const int c = class_of(s, top2);
+                fb = ((qalgo[q] ? NCLASS : 0) + c) * NSUB + slice_of(s, c, top2);
             }
         }
         if (!scatter) {  // the largest task size of the launch (binomial table rows)
@@ -2365,6 +2369,14 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
                            m32, nz, 0u);
     }
 
+    // largest pivoting task the four-word register kernel takes (the rest of
+    // the 129..256 class goes to the block-wide kernel); KC_PIVOT_TOP2
+    // overrides it (129..256)
+    uint32_t pivot_top2 = PIVOT_TOP2;
+    if (const char *e = std::getenv("KC_PIVOT_TOP2")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 129 && v <= 256) pivot_top2 = (uint32_t)v;
+    }
     RunEvents evs;
     if (!scr.fixed && (rc = evs.create())) return rc;
     evs.record(evs.start, s);
@@ -2377,7 +2389,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
         if (!sizes_cached)
             hipLaunchKernelGGL(k_task_size, dim3(gs), dim3(256), 0, s, dq, doff, nq, total, scr.size.as<uint32_t>());
         hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
-                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 0);
+                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 0, pivot_top2);
     }
     KC_HIP(hipGetLastError());
     scr.size_uid = nq == 1 ? qs[0].o->uid : 0;
@@ -2405,7 +2417,7 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
     KC_HIP(hipMemcpyAsync(sub_off, soff, sizeof soff, hipMemcpyHostToDevice, s));
     if (tot > 0)
         hipLaunchKernelGGL(k_classify, dim3(gs), dim3(256), 0, s, dq, dalgo, doff, nq, total, scr.size.as<uint32_t>(),
-                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 1);
+                           sub_cnt, bin_max, sub_pos, sub_off, scr.tasks.as<uint64_t>(), 1, pivot_top2);
     KC_HIP(hipGetLastError());
     evs.record(evs.cls, s);
 
