@@ -261,9 +261,28 @@ def run_config(args):
             ors[o] = G.orient(o)
             log("oriented", o, ors[o].info())
         for k in ks:
+            m_or = ors[o].info()["m"]
+            if args.chunks > 1:
+                # the count as the sum of its task residues mod `chunks`
+                # (kc_count_sampled), one result line per residue, for counts
+                # too long for one call; --chunk-range a:b takes residues a..b-1
+                a0, b0 = (int(x) for x in (args.chunk_range or f"0:{args.chunks}").split(":"))
+                tot, ovf, ms = 0, False, 0.0
+                for off in range(a0, b0):
+                    r = ors[o].count_sampled(k, args.chunks, off, algo, g, part_size=P, check=False)
+                    tot += r.count
+                    ovf |= r.overflow
+                    ms += r.raw["count_ms"]
+                    log(json.dumps({"orient": o, "gran": g, "k": k, "stride": args.chunks, "offset": off,
+                                    "count": r.count, "overflow": r.overflow, "device_ms": round(r.raw["count_ms"], 3)}))
+                ovf |= tot >= 2 ** 64
+                row = {"orient": o, "gran": g, "k": k, "residues": f"{a0}:{b0} of {args.chunks}", "count": tot,
+                       "overflow": ovf, "device_ms": round(ms, 3)}
+                rows.append(row)
+                log(json.dumps(row))
+                continue
             r = ors[o].count(k, algo, g, part_size=P, check=False)
             ms = r.raw["count_ms"]
-            m_or = ors[o].info()["m"]
             row = {"orient": o, "gran": g, "k": k, "count": r.count, "overflow": r.overflow, "device_ms": round(ms, 3),
                    "kcliques_per_s": r.count / (ms / 1e3) if ms else None,
                    "edges_per_s": m_or / (ms / 1e3) if ms else None}
@@ -434,6 +453,8 @@ def main():
                          "lkc: per-vertex counts; formats: converters and the oriented-CSR cache")
     ap.add_argument("--combos", default="degen:edge,degen:node,degree:edge,degree:node")
     ap.add_argument("--c3-ks", default="3,4,5,6,7,8,9,10,11")
+    ap.add_argument("--chunks", type=int, default=1, help="c3: count as the sum over task residues mod N")
+    ap.add_argument("--chunk-range", default="", help="c3 with --chunks: residues a:b only")
     ap.add_argument("--graphs", type=int, default=4096)
     ap.add_argument("--resident", type=int, default=512)
     ap.add_argument("--uploads", type=int, default=64)

@@ -259,3 +259,20 @@ def test_cli_usage_and_argument_errors(tmp_path):
     for args in (["-q", "x8"], ["-o", "full"], ["-m", "lkc"], ["-p", "vertex"]):
         r = subprocess.run([cli, "-g", str(g)] + args, capture_output=True, text=True)
         assert r.returncode != 0, args
+
+
+def test_workload_mix_is_fixed():
+    """The batched and serving workloads are defined by seeds alone: the same
+    specs on every host (bench.py and the GPU tests rely on it)."""
+    from kcgpu_amd.workloads import batch_mix, serve_mix, splitmix64
+    a, b = batch_mix(64), batch_mix(64)
+    assert a == b and len(a) == 64
+    assert all(s["kind"] == ("ba" if i % 2 == 0 else "er") for i, s in enumerate(a))
+    assert all(200 <= s["n"] <= 20000 and 4 <= s["k"] <= 7 for s in a)
+    assert a[0]["seed"] == splitmix64(4 ^ 0) and a[5]["seed"] == splitmix64(4 ^ 5)
+    assert all(3 <= s["attach"] <= 20 for s in a[::2])
+    assert all(s["n"] * 4 // 2 <= s["m"] <= s["n"] * 40 // 2 for s in a[1::2])
+    s1, s2 = serve_mix(40, seed=5), serve_mix(40, seed=5)
+    assert s1 == s2 and [x["kind"] for x in s1[:4]] == ["er", "cl", "rmat", "ba"]
+    # splitmix64 reference values (Steele, Lea, Flood 2014; state 0 and 1)
+    assert splitmix64(0) == 0xE220A8397B1DCDAF and splitmix64(1) == 0x910A2DEC89025CC1
