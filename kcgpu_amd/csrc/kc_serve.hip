@@ -238,7 +238,7 @@ void engine_main(kc_server *s)
     for (;;) {
         std::vector<Ticket> take[kc::OpenEngine::NQ];
         std::vector<kc::OpenQueryIn> in[kc::OpenEngine::NQ];
-        bool any_running = false;
+        bool any_running = false, stopping = false;
         for (int j = 0; j < kc::OpenEngine::NQ; ++j) any_running |= e.running(j);
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -246,6 +246,7 @@ void engine_main(kc_server *s)
             if (inflight.empty() && !any_running) s->cv_work.wait(lk, has_work);
             else s->cv_work.wait_for(lk, std::chrono::microseconds(20), has_work);
             if (s->stop && s->open_pending.empty() && inflight.empty()) break;
+            stopping = s->stop;
             for (int j = 0; j < kc::OpenEngine::NQ; ++j)  // indices restart at 0 when a queue is idle
                 if (!e.qs[j].inflight && e.qs[j].nseg && !e.running(j)) e.reset(j);
             size_t slots = e.free_slots.size();
@@ -310,7 +311,7 @@ void engine_main(kc_server *s)
                 }
                 last_active[j] = Clock::now();
             } else if (run && !e.qs[j].stopped && !e.qs[j].inflight &&
-                       (s->stop || Clock::now() - last_active[j] > STOP_IDLE[j])) {
+                       (stopping || Clock::now() - last_active[j] > STOP_IDLE[j])) {
                 e.stop(j);
             }
         }
