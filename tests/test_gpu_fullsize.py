@@ -149,20 +149,9 @@ def test_batched_mix_4096_graphs_full_size():
 RMAT = dict(edge_factor=16, a=0.57, b=0.19, c=0.19)
 
 
-def test_rmat_scale22_sampled_vs_oracle():
-    """R-MAT scale 22, edge factor 16, (a, b, c, d) = (.57, .19, .19, .05),
-    seed 3, deduplicated, symmetrized, loops dropped (64.2 M edges, degeneracy
-    961): both orientations equal the oracle's entry for entry; strided task
-    samples at k = 3..9 (node and edge tasks to k = 7, edge tasks at k = 8, 9)
-    equal the oracle's counts over the same tasks; the full counts at k = 3..5
-    agree across the four (orientation, granularity) modes."""
-    t0 = time.perf_counter()
+def _rmat22_samples(cases, full_ks=()):
     G = K.Graph.generate(K.GEN_RMAT, seed=3, scale=22, **RMAT)
-    _log("rmat s22", G.info(), f"{time.perf_counter() - t0:.1f}s")
-    cases = {"degen": ((3, "edge", 97, 0), (4, "edge", 97, 1), (5, "node", 97, 2), (6, "node", 499, 3),
-                       (6, "edge", 4999, 4), (7, "node", 2003, 5), (7, "edge", 49999, 6),
-                       (8, "edge", 199999, 7), (9, "edge", 999983, 9)),
-             "degree": ((4, "node", 97, 7), (5, "edge", 997, 8), (6, "node", 997, 9))}
+    _log("rmat s22", G.info())
     full = {}
     for mode, cs in cases.items():
         Og, orp, oc = _oriented_pair(G, mode)
@@ -176,10 +165,30 @@ def test_rmat_scale22_sampled_vs_oracle():
                  f"tasks {r.raw['tasks']} gpu {r.raw['count_ms']:.1f} ms cpu {tc:.1f}s")
             assert r.count == exp["count"] and not r.overflow and not exp["overflow"], (mode, k, gran, stride)
             assert r.raw["tasks"] == exp["tasks"]
-        for k in (3, 4, 5):
+        for k in full_ks:
             for gran in ("node", "edge"):
                 full[(mode, gran, k)] = Og.count(k, "pivot", gran).count
         del Og
+    return full
+
+
+def test_rmat_scale22_sampled_vs_oracle():
+    """R-MAT scale 22, edge factor 16, (a, b, c, d) = (.57, .19, .19, .05),
+    seed 3, deduplicated, symmetrized, loops dropped (64.2 M edges, degeneracy
+    961), degeneracy order: the orientation equals the oracle's entry for
+    entry, and strided task samples at k = 3..7, node and edge tasks, equal
+    the oracle's counts over the same tasks."""
+    _rmat22_samples({"degen": ((3, "edge", 97, 0), (4, "edge", 97, 1), (5, "node", 97, 2), (6, "node", 499, 3),
+                               (6, "edge", 4999, 4), (7, "node", 2003, 5), (7, "edge", 49999, 6))})
+
+
+def test_rmat_scale22_k8_k9_degree_order_and_modes():
+    """The same R-MAT scale 22 graph: edge-task samples at k = 8 and 9
+    (degeneracy order), samples in the degree order, and the full counts at
+    k = 3..5 agree across the four (orientation, granularity) modes."""
+    full = _rmat22_samples({"degen": ((8, "edge", 199999, 7), (9, "edge", 999983, 9)),
+                            "degree": ((4, "node", 97, 7), (5, "edge", 997, 8), (6, "node", 997, 9))},
+                           full_ks=(3, 4, 5))
     for k in (3, 4, 5):
         vals = {v for (_, _, kk), v in full.items() if kk == k}
         _log(f"k={k} full counts over four modes: {vals}")
