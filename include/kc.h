@@ -370,15 +370,16 @@ typedef struct {
 /*
  * Serving loop (no reference counterpart).  Graphs stay oriented in HBM;
  * kc_server_submit queues a query and returns.  Pivoting queries on graphs
- * whose tasks all fit 64 vertices are appended, as they arrive, to the open
- * task queues of one resident persistent launch, whose waves drain them next
- * to the queries already running (continuous batching: a query admitted
- * while others are on the GPU joins the running launch when a query slot
- * and queue room are free; queries of at most 65,536 tasks use an express
- * queue the waves try first).  Other queries are counted by worker threads
- * that batch every pending query fitting the memory budget into persistent
- * launches on their own streams.  A result is exactly what kc_count returns
- * for the query alone.
+ * whose tasks all fit 64 vertices are appended, as they arrive, to the task
+ * queue of a resident persistent launch, whose waves drain them next to the
+ * queries already running (continuous batching: a query admitted while
+ * others are on the GPU joins the running launch when a query slot and
+ * queue room are free).  Queries of at most 65,536 tasks go to an express
+ * launch of 4 waves per CU, larger ones to a bulk launch, so a short query
+ * is not held behind a long one.  Other queries are counted by worker
+ * threads that batch every pending query fitting the memory budget into
+ * persistent launches on their own streams.  A result is exactly what
+ * kc_count returns for the query alone.
  */
 int kc_server_create(const kc_server_config *cfg, kc_server **out);
 /* Orient g and keep it resident; KC_ERR_BUDGET when it does not fit. */

@@ -33,6 +33,10 @@
 // Sizes: tasks are binned by their exact list size (64, 128, 256 vertices in
 // registers, 512, 1024 in LDS, larger in an HBM slice), each class cut into
 // slices drained largest first.
+//
+// Also here: the serving loop's resident launches (k_pivot_open and its host
+// side, OpenEngine), which take tasks from queues the host extends while they
+// run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
