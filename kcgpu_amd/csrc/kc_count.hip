@@ -1728,9 +1728,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         if (!ready) {
             open_flush(oa, w, pend, total, seq);  // nothing held back while the wave waits
             if (lane == 0) {
+                // back off while nothing is published: 1, 2, 4 .. 16 sleeps of
+                // about 3.4 us between polls, so idle waves do not crowd the
+                // device words (and the memory channel) the busy ones use
                 const uint64_t t0 = wall_clock64();
+                uint32_t naps = 1;
                 while (!ready && wall_clock64() - t0 < oa.safety_ticks) {
-                    __builtin_amdgcn_s_sleep(127);
+                    for (uint32_t z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+                    naps = naps < 16 ? 2 * naps : 16;
                     open_refresh(oa);
                     ready = idx < ld_agent(oa.known);
                 }

@@ -222,9 +222,11 @@ void engine_main(kc_server *s)
     hipSetDevice(s->dev);
     kc::OpenEngine &e = *s->eng;
     // an idle launch ends after this long, so its waves free their CUs for the
-    // workers' launches (an express launch kept up for a second between
-    // queries slowed the heavy queries on the workers: serving mix at 8
-    // queries/s, p99 140 ms -> 11.7 s)
+    // workers' launches: a worker's persistent launch is sized for the
+    // device's free slots and completes only when its last block has run, so
+    // an express launch kept up for a second between queries held the heavy
+    // queries on the workers back by that second (serving mix at 8
+    // queries/s: p99 107 ms -> 12.6 s)
     const std::chrono::microseconds STOP_IDLE[kc::OpenEngine::NQ] = {std::chrono::microseconds(2000),
                                                                      std::chrono::microseconds(2000)};
     struct Flight {
