@@ -230,6 +230,15 @@ def test_device_calls_fail_cleanly_without_gpu():
     with pytest.raises(K.KCError) as ei:
         K.Server()
     assert ei.value.code == K.KC_ERR_DEVICE
+    # the device CSR builder and the edge-list orientation need the device too
+    s = np.array([0, 1, 2], np.uint32)
+    d = np.array([1, 2, 0], np.uint32)
+    with pytest.raises(K.KCError) as ei:
+        K.Graph.from_edges_device(s, d)
+    assert ei.value.code == K.KC_ERR_DEVICE
+    with pytest.raises(K.KCError) as ei:
+        K.orient_edges(s, d, "degen")
+    assert ei.value.code == K.KC_ERR_DEVICE
 
 
 def test_autotune_rules():
