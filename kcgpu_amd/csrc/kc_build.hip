@@ -258,7 +258,9 @@ int build_csr_device(DeviceCtx *ctx, hipStream_t s, uint32_t n_in, size_t ne, co
     KC_HIP(hipStreamSynchronize(s));
     if (nbig) {
         const uint32_t words = (n + 63) / 64;
-        const uint32_t nb = std::min<uint32_t>(nbig, (uint32_t)ctx->cu);
+        // one bitmap slice per block, at most 1 GiB of them in all
+        const uint64_t fit = std::max<uint64_t>(1, (1ull << 30) / (8ull * words));
+        const uint32_t nb = (uint32_t)std::min<uint64_t>({(uint64_t)nbig, (uint64_t)ctx->cu, fit});
         DevBuf bm;
         if ((rc = bm.ensure(8ull * words * nb))) return rc;
         hipLaunchKernelGGL(k_row_sort_big, dim3(nb), dim3(BUILD_T), 0, s, off.as<uint32_t>(), big.as<uint32_t>(), nbig,
