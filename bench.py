@@ -465,8 +465,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X only: this build has no multi-GPU path "
-              "(tasks shard by edge with a final 8-byte sum; see DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one MI355X only: there is no multi-GPU path (the tasks would shard by "
+              "edge, each GPU holding the oriented CSR, with one 8-byte sum at the end). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     ks = [int(x) for x in args.ks.split(",")]

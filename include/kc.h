@@ -31,7 +31,7 @@ extern "C" {
 #define KC_ERR_OVERFLOW  6  /* the count is >= 2^64; result.overflow is set */
 #define KC_ERR_BUDGET    7  /* admission refused: footprint above budget */
 #define KC_ERR_NOTFOUND  8  /* unknown graph id / ticket */
-#define KC_ERR_LIMIT     9  /* a fixed limit of this build was exceeded */
+#define KC_ERR_LIMIT     9  /* a fixed limit of the library was exceeded */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *kc_last_error(void);

@@ -188,7 +188,7 @@ int main(int argc, char **argv)
             return 0;
         }
     if (task != "kc" && task != "lkc" && task != "kcore" && task != "cache") {
-        std::fprintf(stderr, "error: -m %s is not part of this build (kc, lkc, kcore, cache, mtx-bel, tsv-bel, "
+        std::fprintf(stderr, "error: -m %s is not supported (kc, lkc, kcore, cache, mtx-bel, tsv-bel, "
                              "tsv-mtx, bel-mtx, txt-bel)\n", task.c_str());
         return 1;
     }
@@ -200,7 +200,7 @@ int main(int argc, char **argv)
     if (orient == "degree") omode = KC_ORIENT_DEGREE;
     else if (orient == "degen") omode = KC_ORIENT_DEGEN;
     else {
-        std::fprintf(stderr, "error: -o %s: this build orients by degree or degen\n", orient.c_str());
+        std::fprintf(stderr, "error: -o %s: the orientation is degree or degen\n", orient.c_str());
         return 1;
     }
     if (task == "kcore") omode = KC_ORIENT_DEGEN;

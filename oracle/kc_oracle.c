@@ -815,7 +815,7 @@ u64 orc_brute_force(u32 n, const u64 *adj, u32 k, int *overflow)
     return (u64)t;
 }
 
-/* orc_threads: OpenMP threads this build would use. */
+/* orc_threads: OpenMP threads the oracle would use. */
 int orc_threads(void)
 {
 #ifdef _OPENMP

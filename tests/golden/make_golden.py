@@ -1,8 +1,8 @@
 """Writes tests/golden/cliques.json: small graphs with their k-clique counts.
 
 Counts (total, and per vertex for k = 3..6) come from networkx (``enumerate_all_cliques``), an implementation
-independent of both the oracle and the HIP kernels, run in the build
-container only; the JSON is what the tests read.  Closed-form families are
+independent of both the oracle and the HIP kernels, run once on a host
+with networkx; the JSON is what the tests read.  Closed-form families are
 added with their formulas so that a mistake in networkx use would also show.
 
 "near_cliques" are larger graphs (200 to 1100 vertices) given by their
