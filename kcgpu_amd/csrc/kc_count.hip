@@ -2755,11 +2755,15 @@ bool OpenEngine::running(int j)
 int OpenEngine::reset(int j)
 {
     if (running(j) || qs[j].inflight) return fail(KC_ERR_ARG, "open queue reset while in use");
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, adm, ctl.as<uint32_t>() + 2 * CTL_WORDS * j,
+    Queue &q = qs[j];
+    // the device words are cleared on the queue's own stream, ahead of its
+    // next launch, and the host does not wait: more streams than hardware
+    // queues share them, and a wait on a stream whose hardware queue holds
+    // another resident launch would last until that launch ends -- which
+    // only this thread's stop can bring about
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, q.run, ctl.as<uint32_t>() + 2 * CTL_WORDS * j,
                        (uint64_t)2 * CTL_WORDS, 0u);
     KC_HIP(hipGetLastError());
-    KC_HIP(hipStreamSynchronize(adm));
-    Queue &q = qs[j];
     __atomic_store_n(q.hnseg, 0ull, __ATOMIC_RELEASE);
     __atomic_store_n(q.htotal, 0ull, __ATOMIC_RELEASE);
     q.total = q.nseg = 0;

@@ -166,7 +166,7 @@ struct OpenEngine {
         uint64_t inflight = 0;
     };
     int dev = -1;
-    hipStream_t adm = nullptr;       // resets
+    hipStream_t adm = nullptr;       // set-up (init) only
     Queue qs[NQ];
     DevBuf qcount, qovf, qdone, ctl, binom, acc;
     uint8_t *pinned = nullptr;       // host: queues, slots, mailboxes
@@ -180,6 +180,7 @@ struct OpenEngine {
     // true while queue j's launch has not ended
     bool running(int j);
     // queue j has no launch and nothing in flight: restart its indices at 0
+    // (never waits on the device)
     int reset(int j);
     // publish the queries as segments of queue j; slot[i] gets query i's slot
     // (~0u when it has no task: its count is 0).  The caller checks room()

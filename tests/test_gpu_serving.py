@@ -18,6 +18,7 @@ def _wait_in_order(srv, tickets, limit_s=120.0):
     order, res = [], {}
     pending = set(tickets)
     t0 = time.perf_counter()
+    t_note = t0
     while pending and time.perf_counter() - t0 < limit_s:
         for t in list(pending):
             r = srv.poll(t)
@@ -25,6 +26,9 @@ def _wait_in_order(srv, tickets, limit_s=120.0):
                 order.append(t)
                 res[t] = r
                 pending.discard(t)
+        if time.perf_counter() - t_note > 10.0:
+            t_note = time.perf_counter()
+            print(f"{t_note - t0:.0f} s: {len(pending)} of {len(tickets)} queries pending", flush=True)
         time.sleep(0.0005)
     assert not pending, f"{len(pending)} queries still pending after {limit_s} s"
     return order, res
@@ -112,3 +116,4 @@ def test_graph_uploaded_mid_launch_and_mixed_modes():
         print(srv.stats(), flush=True)
     finally:
         srv.close()
+
