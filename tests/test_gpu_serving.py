@@ -117,3 +117,54 @@ def test_graph_uploaded_mid_launch_and_mixed_modes():
     finally:
         srv.close()
 
+
+
+def _near_clique_blocks(nblk, size, dens, seed):
+    """nblk disjoint near-cliques of `size` vertices (edge density dens)"""
+    rng = np.random.default_rng(seed)
+    a, b = np.triu_indices(size, 1)
+    es, ed = [], []
+    for i in range(nblk):
+        keep = rng.random(a.size) < dens
+        es.append(a[keep] + i * size)
+        ed.append(b[keep] + i * size)
+    return K.Graph.from_edges(np.concatenate(es).astype(np.uint32), np.concatenate(ed).astype(np.uint32),
+                              nblk * size)
+
+
+def test_express_restarts_while_bulk_runs():
+    """A long query in the bulk resident launch (2,000 near-cliques of 62
+    vertices, 124,000 node tasks, k=10) runs while short queries arrive one
+    at a time with gaps longer than the idle stop, so the express launch
+    stops and restarts (its queue restarting at index 0) several times next
+    to the running bulk launch.  Every count equals kc_count alone."""
+    bulkG = _near_clique_blocks(2000, 62, 0.9, seed=4)
+    shorts = [K.Graph.generate(K.GEN_BA, seed=300 + i, n=3000, attach=5 + i) for i in range(4)]
+    srv = K.Server(workers=2)
+    try:
+        gb = srv.add_graph(bulkG, "degen")
+        gs = [srv.add_graph(G, "degen") for G in shorts]
+        ob = bulkG.orient("degen")
+        os_ = [G.orient("degen") for G in shorts]
+        assert ob.info()["max_out_degree"] <= 64 and ob.info()["n"] > 65536
+        t0 = time.perf_counter()
+        tb = srv.submit(gb, 10, "pivot", "node")
+        launches0 = srv.stats()["resident_launches"]
+        done_short = []
+        for i in range(8):
+            t = srv.submit(gs[i % 4], 4 + i % 3, "pivot", "node")
+            _, r = _wait_in_order(srv, [t], limit_s=60.0)
+            done_short.append((i, r[t], time.perf_counter() - t0))
+            time.sleep(0.005)  # longer than the 2 ms idle stop
+        bulk_running = srv.poll(tb) is None
+        _, rb = _wait_in_order(srv, [tb], limit_s=100.0)
+        st = srv.stats()
+        print(f"bulk done after {time.perf_counter() - t0:.3f} s; short queries done at",
+              [round(x[2], 3) for x in done_short], "bulk still running after them:", bulk_running, st, flush=True)
+        assert st["resident_admitted"] == 9
+        assert st["resident_launches"] - launches0 >= 2
+        assert rb[tb].count == ob.count(10, "pivot", "node").count
+        for i, r, _ in done_short:
+            assert r.count == os_[i % 4].count(4 + i % 3, "pivot", "node").count, i
+    finally:
+        srv.close()
