@@ -2670,8 +2670,11 @@ int run_queries(Scratch &scr, hipStream_t s, const std::vector<QueryRun> &qs, st
 OpenEngine::~OpenEngine()
 {
     if (dev >= 0) hipSetDevice(dev);
-    for (int j = 0; j < NQ; ++j) {
+    // every launch is stopped before any wait: streams sharing a hardware
+    // queue would otherwise wait on a launch not stopped yet
+    for (int j = 0; j < NQ; ++j)
         if (running(j) && !qs[j].stopped) stop(j);
+    for (int j = 0; j < NQ; ++j) {
         if (qs[j].run) hipStreamSynchronize(qs[j].run);
         if (qs[j].done) hipEventDestroy(qs[j].done);
         if (qs[j].run) hipStreamDestroy(qs[j].run);

@@ -91,6 +91,11 @@ struct kc_server {
     std::deque<Ticket> pending;
     std::unordered_map<uint64_t, Done> done;
     uint64_t resident_bytes = 0, inflight_scratch = 0, held_scratch = 0;
+    // removed graphs (and their bytes) waiting to be freed by a worker outside
+    // the lock: hipFree waits for every stream of the device, a resident
+    // launch included, and only the engine thread -- which takes this lock
+    // and must not wait on the device -- can end that launch
+    std::vector<std::pair<kc_oriented *, uint64_t>> free_later;
     uint64_t submitted = 0, completed = 0;             // lifetime, for drain
     uint64_t st_completed = 0, batches = 0, batched_queries = 0;  // since the last stats reset
     std::vector<double> lat;
@@ -118,15 +123,32 @@ void complete(kc_server *s, const Ticket &t, const Done &d)
     s->done[t.id] = d;
     Resident &g = s->graphs[t.gid];
     if (--g.pending == 0 && g.removed) {
-        s->resident_bytes -= g.bytes;
-        kc_oriented_free(g.o);
+        s->free_later.emplace_back(g.o, g.bytes);
         s->graphs.erase(t.gid);
+        s->cv_work.notify_all();
     }
     s->completed++;
     s->st_completed++;
 }
 
 uint64_t oriented_bytes(uint32_t n, uint64_t m) { return 4ull * (n + 1) + 8ull * m + 4ull * n + 4096; }
+
+// free removed graphs (caller holds no lock and is not the engine thread);
+// their bytes leave the budget once they are freed
+void free_graphs(kc_server *s, std::vector<std::pair<kc_oriented *, uint64_t>> &frees)
+{
+    uint64_t bytes = 0;
+    for (auto &f : frees) {
+        kc_oriented_free(f.first);
+        bytes += f.second;
+    }
+    frees.clear();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->resident_bytes -= bytes;
+    }
+    s->cv_work.notify_all();
+}
 
 // transient device memory of kc_orient: full CSR, out-degrees, peel buffers
 uint64_t orient_transient(const kc_graph *g)
@@ -159,14 +181,16 @@ void worker_main(kc_server *s)
     uint64_t held = 0;  // device bytes this worker's scratch holds between batches
     for (;;) {
         std::vector<Ticket> batch;
+        std::vector<std::pair<kc_oriented *, uint64_t>> frees;
         uint64_t batch_scratch = 0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-            if (s->stop && s->pending.empty()) break;
+            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->free_later.empty(); });
+            frees.swap(s->free_later);
+            if (frees.empty() && s->stop && s->pending.empty()) break;
             // admit FIFO while the batch fits next to the resident graphs
             const uint64_t budget = s->cfg.budget_bytes;
-            while (!s->pending.empty() && batch.size() < s->cfg.max_batch) {
+            while (frees.empty() && !s->pending.empty() && batch.size() < s->cfg.max_batch) {
                 const Ticket &t = s->pending.front();
                 const uint64_t need =
                     s->resident_bytes + s->held_scratch + s->inflight_scratch + batch_scratch + t.scratch;
@@ -176,6 +200,10 @@ void worker_main(kc_server *s)
                 s->pending.pop_front();
             }
             s->inflight_scratch += batch_scratch;
+        }
+        if (!frees.empty()) {
+            free_graphs(s, frees);
+            continue;
         }
         std::vector<kc::QueryRun> runs;
         runs.reserve(batch.size());
@@ -430,16 +458,21 @@ int kc_server_add_graph(kc_server *s, const kc_graph *g, int orient_mode, uint32
 int kc_server_remove_graph(kc_server *s, uint32_t gid)
 {
     if (!s) return fail(KC_ERR_ARG, "kc_server_remove_graph: null server");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->graphs.find(gid);
-    if (it == s->graphs.end() || it->second.removed) return fail(KC_ERR_NOTFOUND, "no graph %u", gid);
-    if (it->second.pending == 0) {
-        s->resident_bytes -= it->second.bytes;
-        kc_oriented_free(it->second.o);
-        s->graphs.erase(it);
-    } else {
-        it->second.removed = true;  // freed by the worker after its last query
+    std::vector<std::pair<kc_oriented *, uint64_t>> frees;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->graphs.find(gid);
+        if (it == s->graphs.end() || it->second.removed) return fail(KC_ERR_NOTFOUND, "no graph %u", gid);
+        if (it->second.pending == 0) {
+            frees.emplace_back(it->second.o, it->second.bytes);
+            s->graphs.erase(it);
+        } else {
+            it->second.removed = true;  // freed by a worker after its last query
+        }
     }
+    // outside the lock: the free waits for the device, resident launches
+    // included, which the engine thread ends while this thread waits
+    if (!frees.empty()) free_graphs(s, frees);
     return KC_OK;
 }
 
@@ -583,6 +616,7 @@ void kc_server_destroy(kc_server *s)
     for (auto &t : s->workers) t.join();
     if (s->engine.joinable()) s->engine.join();
     s->eng.reset();  // waits for the resident launch to end
+    for (auto &f : s->free_later) kc_oriented_free(f.first);
     for (auto &kv : s->graphs) kc_oriented_free(kv.second.o);
     delete s;
 }

@@ -168,3 +168,48 @@ def test_express_restarts_while_bulk_runs():
             assert r.count == os_[i % 4].count(4 + i % 3, "pivot", "node").count, i
     finally:
         srv.close()
+
+
+def test_remove_graph_while_its_query_runs():
+    """A graph removed while its query is in the bulk resident launch: the
+    query still completes with kc_count's count, the graph's memory is
+    freed (resident bytes fall back) by a worker after that query, and other
+    queries keep completing meanwhile; a graph removed with nothing in
+    flight is freed at once."""
+    bulkG = _near_clique_blocks(1100, 62, 0.9, seed=5)  # 68,200 node tasks: the bulk queue
+    other = K.Graph.generate(K.GEN_BA, seed=400, n=3000, attach=6)
+    idle = K.Graph.generate(K.GEN_BA, seed=401, n=3000, attach=7)
+    srv = K.Server(workers=2)
+    try:
+        go = srv.add_graph(other, "degen")
+        base = srv.stats()["resident_bytes"]
+        gi = srv.add_graph(idle, "degen")
+        srv.remove_graph(gi)  # nothing in flight: freed by the call
+        assert srv.stats()["resident_bytes"] == base
+        gb = srv.add_graph(bulkG, "degen")
+        assert srv.stats()["resident_bytes"] > base
+        ob, oo = bulkG.orient("degen"), other.orient("degen")
+        tb = srv.submit(gb, 10, "pivot", "node")
+        srv.remove_graph(gb)  # its query is in flight
+        with pytest.raises(K.KCError) as ei:
+            srv.submit(gb, 4)
+        assert ei.value.code == K.KC_ERR_NOTFOUND
+        shorts = []
+        for i in range(6):
+            t = srv.submit(go, 4 + i % 3, "pivot", "node")
+            _, r = _wait_in_order(srv, [t], limit_s=60.0)
+            shorts.append((i, r[t]))
+            time.sleep(0.005)
+        _, rb = _wait_in_order(srv, [tb], limit_s=100.0)
+        assert rb[tb].count == ob.count(10, "pivot", "node").count
+        for i, r in shorts:
+            assert r.count == oo.count(4 + i % 3, "pivot", "node").count, i
+        t0 = time.perf_counter()
+        while srv.stats()["resident_bytes"] != base and time.perf_counter() - t0 < 30.0:
+            time.sleep(0.01)
+        st = srv.stats()
+        print(f"removed graph freed {time.perf_counter() - t0:.3f} s after its query", st, flush=True)
+        assert st["resident_bytes"] == base
+        assert st["resident_admitted"] == 7
+    finally:
+        srv.close()
