@@ -384,7 +384,10 @@ typedef struct {
 int kc_server_create(const kc_server_config *cfg, kc_server **out);
 /* Orient g and keep it resident; KC_ERR_BUDGET when it does not fit. */
 int kc_server_add_graph(kc_server *s, const kc_graph *g, int orient_mode, uint32_t *gid);
-/* Drop a graph (after its queued queries finish). */
+/* Drop a graph: new queries on it fail with KC_ERR_NOTFOUND.  With nothing
+ * in flight its memory is freed before the call returns (the free waits for
+ * the device's running launches); otherwise a worker frees it after its
+ * last query, and its bytes stay in the budget until then. */
 int kc_server_remove_graph(kc_server *s, uint32_t gid);
 /* Queue a query; returns at once with a ticket. */
 int kc_server_submit(kc_server *s, uint32_t gid, const kc_query *q, uint64_t *ticket);
