@@ -436,6 +436,35 @@ def cpu_baseline(K, Og, ks, m_or, stride, repeats):
             "parity": {"stride": stride, "offset": 0, "per_k": parity, "equal": True}}
 
 
+def sum_residues(paths):
+    """The full count from the per-residue lines of `--workload c3 --chunks N`
+    runs (one JSON line per residue, possibly over several files and calls):
+    one result per (orientation, granularity, k, N), with the residues that are
+    missing or repeated with different counts named; the sum overflows when a
+    residue did or the total reaches 2^64."""
+    res = {}
+    for path in paths:
+        with open(path) as f:
+            for line in f:
+                if not line.startswith('{"orient"') or '"offset"' not in line:
+                    continue
+                d = json.loads(line)
+                key = (d["orient"], d["gran"], d["k"], d["stride"])
+                prev = res.setdefault(key, {}).get(d["offset"])
+                cur = (d["count"], bool(d["overflow"]), d["device_ms"])
+                if prev is not None and prev[:2] != cur[:2]:
+                    raise ValueError(f"residue {d['offset']} of {key} has two counts: {prev[0]} and {cur[0]}")
+                res[key][d["offset"]] = cur
+    out = []
+    for (o, g, k, n), r in sorted(res.items()):
+        tot = sum(c for c, _, _ in r.values())
+        out.append({"orient": o, "gran": g, "k": k, "residues_mod": n, "have": len(r),
+                    "missing": sorted(set(range(n)) - set(r)), "count": tot,
+                    "overflow": any(v for _, v, _ in r.values()) or tot >= 2 ** 64,
+                    "device_s": round(sum(m for _, _, m in r.values()) / 1e3, 3)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -455,6 +484,8 @@ def main():
     ap.add_argument("--c3-ks", default="3,4,5,6,7,8,9,10,11")
     ap.add_argument("--chunks", type=int, default=1, help="c3: count as the sum over task residues mod N")
     ap.add_argument("--chunk-range", default="", help="c3 with --chunks: residues a:b only")
+    ap.add_argument("--sum-residues", nargs="+", default=None, metavar="LOG",
+                    help="sum the residue lines of --chunks runs saved in LOG files (no GPU)")
     ap.add_argument("--graphs", type=int, default=4096)
     ap.add_argument("--resident", type=int, default=512)
     ap.add_argument("--uploads", type=int, default=64)
@@ -464,6 +495,11 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=256)
     args = ap.parse_args()
+    if args.sum_residues:
+        rows = sum_residues(args.sum_residues)
+        for r in rows:
+            print(json.dumps(r), flush=True)
+        sys.exit(0 if rows and all(not r["missing"] for r in rows) else 1)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X only: there is no multi-GPU path (the tasks would shard by "
               "edge, each GPU holding the oriented CSR, with one 8-byte sum at the end). Run with --gpus 1.",

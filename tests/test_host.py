@@ -285,3 +285,38 @@ def test_workload_mix_is_fixed():
     assert s1 == s2 and [x["kind"] for x in s1[:4]] == ["er", "cl", "rmat", "ba"]
     # splitmix64 reference values (Steele, Lea, Flood 2014; state 0 and 1)
     assert splitmix64(0) == 0xE220A8397B1DCDAF and splitmix64(1) == 0x910A2DEC89025CC1
+
+
+def _bench_module():
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py")
+    spec = importlib.util.spec_from_file_location("kc_bench", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_sum_residues_of_chunked_counts(tmp_path):
+    """bench.py --sum-residues: residue lines over several files add up per
+    (orientation, granularity, k, stride); a residue seen twice with the same
+    count counts once, missing residues are named, a residue with two
+    different counts is an error, and a total of 2^64 or more is an overflow."""
+    import json
+    B = _bench_module()
+
+    def line(o, off, c, n=4, ovf=False):
+        return json.dumps({"orient": o, "gran": "edge", "k": 8, "stride": n, "offset": off, "count": c,
+                           "overflow": ovf, "device_ms": 10.0}) + "\n"
+    a, b = tmp_path / "a.log", tmp_path / "b.log"
+    a.write_text("noise\n" + line("degen", 0, 5) + line("degen", 1, 7) + line("degree", 0, 2 ** 63))
+    b.write_text(line("degen", 1, 7) + line("degen", 2, 11) + line("degen", 3, 13) + line("degree", 1, 2 ** 63)
+                 + '{"orient": "degen", "gran": "edge", "k": 8, "residues": "0:4 of 4", "count": 36}\n')
+    rows = {r["orient"]: r for r in B.sum_residues([str(a), str(b)])}
+    assert rows["degen"]["count"] == 36 and rows["degen"]["missing"] == [] and not rows["degen"]["overflow"]
+    assert rows["degen"]["have"] == 4 and rows["degen"]["device_s"] == 0.04
+    assert rows["degree"]["missing"] == [2, 3] and rows["degree"]["overflow"]
+    c = tmp_path / "c.log"
+    c.write_text(line("degen", 0, 6))
+    with pytest.raises(ValueError):
+        B.sum_residues([str(a), str(c)])
